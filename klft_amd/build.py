@@ -1,0 +1,98 @@
+"""Build recipe for libklft_hip.so (gfx950) and the metropolis / hmc drivers.
+
+Every HIP translation unit under csrc/ is compiled for gfx950 with hipcc in
+parallel; the objects are linked into klft_amd/libklft_hip.so (in-tree, so it
+travels with the source tree).  The drivers link against it with an rpath of
+$ORIGIN.
+
+Usage: python -m klft_amd.build [--jobs N]
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+OBJ = HERE / "_build"
+LIB = HERE / "libklft_hip.so"
+INCLUDE = HERE.parent / "include"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# Packed FP32 instructions are switched off for every device pass.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INCLUDE}",
+          "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
+
+HIP_SOURCES = ["core.hip", "metropolis.hip", "hmc.hip", "batch.hip"]
+HOST_SOURCES = ["host/yaml_lite.cpp", "host/observables.cpp", "host/driver.cpp",
+                "host/tuner.cpp"]
+TOOLS = {"metropolis": "tools/metropolis_main.cpp", "hmc": "tools/hmc_main.cpp",
+         "plaq_bench": "tools/plaq_bench.cpp", "wloop_temporal_bench": "tools/wloop_bench.cpp",
+         "metropolis_bench": "tools/metropolis_bench.cpp"}
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("command failed:\n" + " ".join(cmd) + "\n" + r.stdout)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if out:
+        sys.stderr.write(out + "\n")
+
+
+def _compile(src: Path, obj: Path) -> Path:
+    obj.parent.mkdir(parents=True, exist_ok=True)
+    if src.suffix == ".hip":
+        cmd = [HIPCC, *COMMON, *DEVICE_FLAGS, "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", f"-I{INCLUDE}", "-Wall",
+               "-c", "-x", "c++", str(src), "-o", str(obj)]
+    _run(cmd)
+    return obj
+
+
+def build(jobs: int | None = None, tools: bool = True) -> Path:
+    """Compile every HIP/C++ source and link libklft_hip.so (always rebuilds)."""
+    if not Path(HIPCC).exists() and shutil.which("hipcc") is None:
+        raise RuntimeError("hipcc not found")
+    srcs = [CSRC / s for s in HIP_SOURCES if (CSRC / s).exists()]
+    srcs += [CSRC / s for s in HOST_SOURCES if (CSRC / s).exists()]
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(_compile, s, OBJ / (s.relative_to(CSRC).as_posix().replace("/", "_") + ".o"))
+                for s in srcs]
+        for f in futs:
+            objs.append(f.result())
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, LIB)
+    if tools:
+        for name, src in TOOLS.items():
+            s = CSRC / src
+            if not s.exists():
+                continue
+            _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", "-x", "c++", str(s), "-o",
+                  str(HERE / name), f"-L{HERE}", "-lklft_hip", "-Wl,-rpath,$ORIGIN"])
+    return LIB
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--no-tools", action="store_true")
+    a = ap.parse_args()
+    print(build(a.jobs, tools=not a.no_tools))
+
+
+if __name__ == "__main__":
+    main()

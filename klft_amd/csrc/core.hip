@@ -1,0 +1,805 @@
+// libklft_hip core: device context, gauge-field handles, host interchange,
+// field initialisation, plaquette, staple field and Wilson loops.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace kd;
+namespace ki = klft_internal;
+
+// ============================================================================
+// context and errors
+// ============================================================================
+namespace klft_internal {
+
+static thread_local std::string g_err;
+static std::mutex g_init_mu;
+static bool g_inited = false;
+static int g_device = 0;
+static hipStream_t g_stream = nullptr;
+
+int set_error(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return set_error(KLFT_E_HIP, "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+}
+
+static int init_device(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return set_error(KLFT_E_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return set_error(KLFT_E_ARG, "device %d out of range", device);
+  KLFT_HIP(hipSetDevice(device));
+  KLFT_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_device = device;
+  g_inited = true;
+  return KLFT_OK;
+}
+
+int ensure_init() {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_inited) {
+    KLFT_HIP(hipSetDevice(g_device));
+    return KLFT_OK;
+  }
+  return init_device(0);
+}
+
+hipStream_t stream() { return g_stream; }
+
+int sync_stream() {
+  KLFT_HIP(hipStreamSynchronize(g_stream));
+  return KLFT_OK;
+}
+
+Geom make_geom(int nd, const int *dims) {
+  Geom g{};
+  g.nd = nd;
+  g.V = 1;
+  g.Vh = 1;
+  for (int d = 0; d < 4; ++d) {
+    g.L[d] = d < nd ? dims[d] : 1;
+    g.H[d] = d < nd ? dims[d] / 2 : 1;
+  }
+  for (int d = 0; d < nd; ++d) {
+    g.V *= g.L[d];
+    g.Vh *= g.H[d];
+  }
+  for (int d = 0; d < 4; ++d) {
+    g.hs[d] = 0;
+    g.ls[d] = 0;
+  }
+  g.hs[nd - 1] = 1;
+  g.ls[nd - 1] = 1;
+  for (int d = nd - 2; d >= 0; --d) {
+    g.hs[d] = g.hs[d + 1] * g.H[d + 1];
+    g.ls[d] = g.ls[d + 1] * g.L[d + 1];
+  }
+  return g;
+}
+
+int check_field(const klft_field_s *f) {
+  if (!f || !f->links) return set_error(KLFT_E_ARG, "null or destroyed field handle");
+  return KLFT_OK;
+}
+
+}  // namespace klft_internal
+
+extern "C" int klft_init(int device) {
+  std::lock_guard<std::mutex> lk(ki::g_init_mu);
+  if (ki::g_inited) {
+    if (device == ki::g_device) return KLFT_OK;
+    return ki::set_error(KLFT_E_STATE, "library already initialised on device %d", ki::g_device);
+  }
+  return ki::init_device(device);
+}
+
+extern "C" const char *klft_last_error(void) { return ki::g_err.c_str(); }
+
+extern "C" const char *klft_version(void) { return "klft_hip 1.0 gfx950"; }
+
+// ============================================================================
+// kernels: interchange and initialisation
+// ============================================================================
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_upload(Links<R, NC, COMP> f, Geom g,
+                                                 const double *__restrict__ host_img) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;  // lexicographic
+  if (s >= g.V) return;
+  int x[ND];
+  lex_coords<ND>(g, s, x);
+  const long long d = dev_index<ND>(g, x);
+#pragma unroll
+  for (int mu = 0; mu < ND; ++mu) {
+    Mat<double, NC> u;
+    const double *p = host_img + 2 * ((size_t)s * ND + mu) * NC * NC;
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) u.m[i][j] = mk(p[2 * (i * NC + j)], p[2 * (i * NC + j) + 1]);
+    f.store(d, mu, convert<R>(u));
+  }
+}
+
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_download(Links<R, NC, COMP> f, Geom g,
+                                                   double *__restrict__ host_img) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.V) return;
+  int x[ND];
+  lex_coords<ND>(g, s, x);
+  const long long d = dev_index<ND>(g, x);
+#pragma unroll
+  for (int mu = 0; mu < ND; ++mu) {
+    const Mat<R, NC> u = f.load(d, mu);
+    double *p = host_img + 2 * ((size_t)s * ND + mu) * NC * NC;
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        p[2 * (i * NC + j)] = (double)u.m[i][j].re;
+        p[2 * (i * NC + j) + 1] = (double)u.m[i][j].im;
+      }
+  }
+}
+
+// kind: 0 identity, 1 randSUN(delta), 2 exp(i eps Q), 3 Haar
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_init(Links<R, NC, COMP> f, Geom g, int kind,
+                                               double param, uint64_t seed) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;  // lexicographic
+  if (s >= g.V) return;
+  int x[ND];
+  lex_coords<ND>(g, s, x);
+  const long long d = dev_index<ND>(g, x);
+  constexpr int NG = Ngen<NC>::value;
+  for (int mu = 0; mu < ND; ++mu) {
+    Mat<double, NC> u;
+    if (kind == 0) {
+      u = identity<double, NC>();
+    } else if (kind == 1) {
+      klft_rng::Stream st(seed, 5, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
+      rand_sun(st, param, u);
+    } else if (kind == 2) {
+      klft_rng::Stream st(seed, klft_rng::TAG_ALGEBRA, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
+      double h[NG + 1];
+#pragma unroll
+      for (int a = 0; a < NG; a += 2) {
+        const double u1 = 1.0 - st.u01(), u2 = st.u01();
+        const double r = sqrt(-2.0 * log(u1)), t = 2.0 * KLFT_PI * u2;
+        h[a] = r * cos(t);
+        h[a + 1] = r * sin(t);
+      }
+      double v[NG];
+#pragma unroll
+      for (int a = 0; a < NG; ++a) v[a] = param * h[a] * 0.5;
+      u = exp_iq(v, (Mat<double, NC> *)nullptr);
+    } else {
+      klft_rng::Stream st(seed, klft_rng::TAG_HAAR, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
+      double z[2 * NC * NC + 2];
+#pragma unroll
+      for (int a = 0; a < 2 * NC * NC; a += 2) {
+        const double u1 = 1.0 - st.u01(), u2 = st.u01();
+        const double r = sqrt(-2.0 * log(u1)), t = 2.0 * KLFT_PI * u2;
+        z[a] = r * cos(t);
+        z[a + 1] = r * sin(t);
+      }
+      if constexpr (NC == 1) {
+        u.m[0][0] = mk(z[0], z[1]);
+        restore_sun(u);
+      } else if constexpr (NC == 2) {
+        u.m[0][0] = mk(z[0], z[3]);
+        u.m[0][1] = mk(z[2], z[1]);
+        complete_row1(u);
+        restore_sun(u);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) u.m[i][j] = mk(z[2 * (3 * i + j)], z[2 * (3 * i + j) + 1]);
+        // Gram-Schmidt in FP64, third row conj(r0 x r1)
+        double n0 = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) n0 += u.m[0][j].re * u.m[0][j].re + u.m[0][j].im * u.m[0][j].im;
+        n0 = 1.0 / sqrt(n0);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) u.m[0][j] = scale(n0, u.m[0][j]);
+        cx<double> p = cmul_(u.m[0][0], u.m[1][0]) + cmul_(u.m[0][1], u.m[1][1]) +
+                       cmul_(u.m[0][2], u.m[1][2]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) u.m[1][j] = u.m[1][j] - p * u.m[0][j];
+        double n1 = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) n1 += u.m[1][j].re * u.m[1][j].re + u.m[1][j].im * u.m[1][j].im;
+        n1 = 1.0 / sqrt(n1);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) u.m[1][j] = scale(n1, u.m[1][j]);
+        complete_row2(u);
+      }
+    }
+    f.store(d, mu, convert<R>(u));
+  }
+}
+
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_reunit(Links<R, NC, COMP> f, Geom g) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.V) return;
+#pragma unroll
+  for (int mu = 0; mu < ND; ++mu) {
+    Mat<R, NC> u = f.load(s, mu);
+    reunitarize(u);
+    f.store(s, mu, u);
+  }
+}
+
+template <typename RO, int NC, bool CO, typename RI, bool CI, int ND>
+__global__ __launch_bounds__(256) void k_convert(Links<RO, NC, CO> o, Links<RI, NC, CI> in, Geom g) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.V) return;
+#pragma unroll
+  for (int mu = 0; mu < ND; ++mu) o.store(s, mu, convert<RO>(in.load(s, mu)));
+}
+
+// ============================================================================
+// field lifecycle
+// ============================================================================
+extern "C" int klft_field_create(int nd, int nc, const int *dims, int storage, klft_field_t *out) {
+  if (!out || !dims) return ki::set_error(KLFT_E_ARG, "null argument");
+  *out = nullptr;
+  if (nd < 2 || nd > 4) return ki::set_error(KLFT_E_ARG, "nd=%d unsupported (2..4)", nd);
+  if (nc < 1 || nc > 3) return ki::set_error(KLFT_E_ARG, "Nc=%d unsupported (1..3)", nc);
+  if (storage < KLFT_STORAGE_F64 || storage > KLFT_STORAGE_F32_ROWS)
+    return ki::set_error(KLFT_E_ARG, "storage=%d unsupported", storage);
+  long long V = 1;
+  for (int d = 0; d < nd; ++d) {
+    if (dims[d] < 2 || (dims[d] & 1))
+      return ki::set_error(KLFT_E_ARG, "extent L%d=%d must be even and >= 2", d, dims[d]);
+    V *= dims[d];
+  }
+  if (V >= (1LL << 32)) return ki::set_error(KLFT_E_ARG, "volume %lld exceeds 2^32 sites", V);
+  int rc = ki::ensure_init();
+  if (rc) return rc;
+  klft_field_s *f = new klft_field_s();
+  f->nd = nd;
+  f->nc = nc;
+  f->storage = storage;
+  for (int d = 0; d < nd; ++d) f->dims[d] = dims[d];
+  f->g = ki::make_geom(nd, dims);
+  const bool comp = storage == KLFT_STORAGE_F32_ROWS && nc > 1;
+  f->ncomp = comp ? (nc - 1) * nc : nc * nc;
+  f->cbytes = storage == KLFT_STORAGE_F64 ? 16 : 8;
+  f->link_bytes = (size_t)V * nd * f->ncomp * f->cbytes;
+  hipError_t e = hipMalloc(&f->links, f->link_bytes);
+  if (e != hipSuccess) {
+    delete f;
+    return ki::set_error(KLFT_E_NOMEM, "hipMalloc of %zu link bytes failed", (size_t)V * nd * 16);
+  }
+  size_t ws = sizeof(double) * ki::kMaxPartials + 256 + 16 * sizeof(double) + 8 * sizeof(unsigned long long);
+  char *w = nullptr;
+  e = hipMalloc((void **)&w, ws);
+  if (e != hipSuccess) {
+    (void)hipFree(f->links);
+    delete f;
+    return ki::set_error(KLFT_E_NOMEM, "workspace allocation failed");
+  }
+  f->partials = (double *)w;
+  f->ticket = (unsigned int *)(w + sizeof(double) * ki::kMaxPartials);
+  f->result = (double *)(w + sizeof(double) * ki::kMaxPartials + 256);
+  f->counter = (unsigned long long *)(w + sizeof(double) * ki::kMaxPartials + 256 + 16 * sizeof(double));
+  e = hipMemset(w, 0, ws);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&f->host_result, 16 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipFree(w);
+    (void)hipFree(f->links);
+    delete f;
+    return ki::hip_fail(e, "workspace init");
+  }
+  *out = f;
+  return KLFT_OK;
+}
+
+extern "C" int klft_field_destroy(klft_field_t f) {
+  if (!f) return KLFT_OK;
+  (void)ki::sync_stream();
+  if (f->links) (void)hipFree(f->links);
+  if (f->partials) (void)hipFree(f->partials);
+  if (f->host_result) (void)hipHostFree(f->host_result);
+  f->links = nullptr;
+  delete f;
+  return KLFT_OK;
+}
+
+extern "C" int klft_field_info(klft_field_t f, int *nd, int *nc, int *dims, int *storage) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (nd) *nd = f->nd;
+  if (nc) *nc = f->nc;
+  if (dims)
+    for (int d = 0; d < f->nd; ++d) dims[d] = f->dims[d];
+  if (storage) *storage = f->storage;
+  return KLFT_OK;
+}
+
+static int launch_init(klft_field_t f, int kind, double param, uint64_t seed) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    hipLaunchKernelGGL((k_init<R, NC, COMP, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256), 0,
+                       ki::stream(), ki::links_of<R, NC, COMP>(f), f->g, kind, param, seed);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+  if (rc) return rc;
+  return ki::sync_stream();
+}
+
+extern "C" int klft_field_set_identity(klft_field_t f) { return launch_init(f, 0, 0.0, 0); }
+extern "C" int klft_field_set_random_sun(klft_field_t f, double delta, uint64_t seed) {
+  return launch_init(f, 1, delta, seed);
+}
+extern "C" int klft_field_set_random_algebra(klft_field_t f, double eps, uint64_t seed) {
+  return launch_init(f, 2, eps, seed);
+}
+extern "C" int klft_field_set_haar(klft_field_t f, uint64_t seed) { return launch_init(f, 3, 0.0, seed); }
+
+extern "C" int klft_field_upload(klft_field_t f, const double *host) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (!host) return ki::set_error(KLFT_E_ARG, "null host buffer");
+  const size_t n = (size_t)f->g.V * f->nd * f->nc * f->nc * 2 * sizeof(double);
+  double *stage = nullptr;
+  KLFT_HIP(hipMalloc(&stage, n));
+  hipError_t e = hipMemcpyAsync(stage, host, n, hipMemcpyHostToDevice, ki::stream());
+  if (e == hipSuccess) {
+    rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+      KLFT_UNPACK(rt, nct, ct, ndt);
+      hipLaunchKernelGGL((k_upload<R, NC, COMP, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256), 0,
+                         ki::stream(), ki::links_of<R, NC, COMP>(f), f->g, stage);
+      KLFT_HIP(hipGetLastError());
+      return KLFT_OK;
+    });
+    e = hipStreamSynchronize(ki::stream());
+  }
+  (void)hipFree(stage);
+  if (e != hipSuccess) return ki::hip_fail(e, "upload");
+  return rc;
+}
+
+extern "C" int klft_field_download(klft_field_t f, double *host) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (!host) return ki::set_error(KLFT_E_ARG, "null host buffer");
+  const size_t n = (size_t)f->g.V * f->nd * f->nc * f->nc * 2 * sizeof(double);
+  double *stage = nullptr;
+  KLFT_HIP(hipMalloc(&stage, n));
+  rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    hipLaunchKernelGGL((k_download<R, NC, COMP, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256), 0,
+                       ki::stream(), ki::links_of<R, NC, COMP>(f), f->g, stage);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipMemcpyAsync(host, stage, n, hipMemcpyDeviceToHost, ki::stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ki::stream());
+  (void)hipFree(stage);
+  if (e != hipSuccess) return ki::hip_fail(e, "download");
+  return rc;
+}
+
+extern "C" int klft_field_copy(klft_field_t dst, klft_field_t src) {
+  int rc = ki::check_field(dst);
+  if (!rc) rc = ki::check_field(src);
+  if (rc) return rc;
+  if (dst->nd != src->nd || dst->nc != src->nc)
+    return ki::set_error(KLFT_E_ARG, "copy between fields of different shape");
+  for (int d = 0; d < src->nd; ++d)
+    if (dst->dims[d] != src->dims[d]) return ki::set_error(KLFT_E_ARG, "copy between different extents");
+  if (dst->storage == src->storage) {
+    KLFT_HIP(hipMemcpyAsync(dst->links, src->links, src->link_bytes, hipMemcpyDeviceToDevice, ki::stream()));
+    return ki::sync_stream();
+  }
+  // conversion: dispatch on the source, then on the destination storage
+  rc = ki::dispatch(src->storage, src->nc, src->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    using RI = typename decltype(rt)::type;
+    constexpr int NC = decltype(nct)::value;
+    constexpr bool CI = decltype(ct)::value != 0;
+    constexpr int ND = decltype(ndt)::value;
+    auto in = ki::links_of<RI, NC, CI>(src);
+    const unsigned nb = ki::blocks_for(src->g.V, 256);
+    switch (dst->storage) {
+      case KLFT_STORAGE_F64:
+        hipLaunchKernelGGL((k_convert<double, NC, false, RI, CI, ND>), dim3(nb), dim3(256), 0, ki::stream(),
+                           ki::links_of<double, NC, false>(dst), in, src->g);
+        break;
+      case KLFT_STORAGE_F32:
+        hipLaunchKernelGGL((k_convert<float, NC, false, RI, CI, ND>), dim3(nb), dim3(256), 0, ki::stream(),
+                           ki::links_of<float, NC, false>(dst), in, src->g);
+        break;
+      default:
+        hipLaunchKernelGGL((k_convert<float, NC, true, RI, CI, ND>), dim3(nb), dim3(256), 0, ki::stream(),
+                           ki::links_of<float, NC, true>(dst), in, src->g);
+        break;
+    }
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+  if (rc) return rc;
+  return ki::sync_stream();
+}
+
+int klft_internal::enqueue_reunit(klft_field_t f) {
+  return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    hipLaunchKernelGGL((k_reunit<R, NC, COMP, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256), 0,
+                       ki::stream(), ki::links_of<R, NC, COMP>(f), f->g);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+}
+
+extern "C" int klft_field_reunitarize(klft_field_t f) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  rc = ki::enqueue_reunit(f);
+  if (rc) return rc;
+  return ki::sync_stream();
+}
+
+// ============================================================================
+// plaquette (GaugePlaquette.hpp:36-135)
+// ============================================================================
+// p(x) = sum_{mu<nu} Re Tr[(U_mu(x) U_nu(x+mu)) (U_nu(x) U_mu(x+nu))^+]
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ double plaq_at(const Links<R, NC, COMP> &f, const Geom &g, const int *x,
+                                          long long s) {
+  double p = 0.0;
+#pragma unroll
+  for (int mu = 0; mu < ND - 1; ++mu) {
+    int xm[ND];
+    copy_x<ND>(x, xm);
+    xm[mu] = up(x[mu], g.L[mu]);
+    const long long sm = dev_index<ND>(g, xm);
+    const Mat<R, NC> umu = f.load(s, mu);
+#pragma unroll
+    for (int nu = mu + 1; nu < ND; ++nu) {
+      int xn[ND];
+      copy_x<ND>(x, xn);
+      xn[nu] = up(x[nu], g.L[nu]);
+      const Mat<R, NC> lmu = mul(umu, f.load(sm, nu));
+      const Mat<R, NC> lnu = mul(f.load(s, nu), f.load(dev_index<ND>(g, xn), mu));
+      p += (double)retr_mul_ad(lmu, lnu);
+    }
+  }
+  return p;
+}
+
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_plaquette(Links<R, NC, COMP> f, Geom g, double *partials,
+                                                    unsigned int *ticket, double *out,
+                                                    double *per_site) {
+  __shared__ double lds[8];
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  int x[ND];
+  long long s;
+  double v = 0.0;
+  if (chunk_site<ND>(g, lb, x, &s)) {
+    v = plaq_at<R, NC, COMP, ND>(f, g, x, s);
+    if (per_site) per_site[lex_index<ND>(g, x)] = v;
+  }
+  const double bt = block_sum<256>(v, lds);
+  grid_finish<256>(bt, partials, ticket, out, lds);
+}
+
+namespace klft_internal {
+// raw (unnormalised) plaquette sum into f->result[slot]; optional per-site
+int plaquette_raw(klft_field_t f, int slot, double *dev_per_site) {
+  return dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    const unsigned nb = blocks_for(chunk_threads<ND>(f->g), 256);
+    if (nb > (unsigned)kMaxPartials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
+    hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND>), dim3(nb), dim3(256), 0, stream(),
+                       links_of<R, NC, COMP>(f), f->g, f->partials, f->ticket, f->result + slot,
+                       dev_per_site);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+}
+int read_results(klft_field_t f, int first, int n, double *out) {
+  KLFT_HIP(hipMemcpyAsync(f->host_result, f->result + first, n * sizeof(double), hipMemcpyDeviceToHost,
+                          stream()));
+  KLFT_HIP(hipStreamSynchronize(stream()));
+  for (int i = 0; i < n; ++i) out[i] = f->host_result[i];
+  return KLFT_OK;
+}
+}  // namespace klft_internal
+
+extern "C" int klft_plaquette(klft_field_t f, int normalize, double *out, double *per_site) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (!out) return ki::set_error(KLFT_E_ARG, "null output");
+  double *dps = nullptr;
+  if (per_site) KLFT_HIP(hipMalloc(&dps, sizeof(double) * f->g.V));
+  rc = ki::plaquette_raw(f, 0, dps);
+  double sum = 0.0;
+  if (!rc) rc = ki::read_results(f, 0, 1, &sum);
+  if (!rc && per_site) {
+    hipError_t e = hipMemcpy(per_site, dps, sizeof(double) * f->g.V, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = ki::hip_fail(e, "per-site copy");
+  }
+  if (dps) (void)hipFree(dps);
+  if (rc) return rc;
+  if (normalize) {  // GaugePlaquette.hpp:125-132
+    double norm = 1.0;
+    for (int d = 0; d < f->nd; ++d) norm *= (double)f->dims[d];
+    norm *= (double)((f->nd * (f->nd - 1) / 2) * f->nc);
+    sum /= norm;
+  }
+  *out = sum;
+  return KLFT_OK;
+}
+
+// ============================================================================
+// staple field (Gauge_Util.hpp:27-97)
+// ============================================================================
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_staple_field(Links<R, NC, COMP> f, Links<R, NC, false> o,
+                                                       Geom g) {
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  int x[ND];
+  long long s;
+  if (!chunk_site<ND>(g, lb, x, &s)) return;
+#pragma unroll 1
+  for (int mu = 0; mu < ND; ++mu) o.store(s, mu, staple<R, NC, COMP, ND>(f, g, x, mu));
+}
+
+extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
+  int rc = ki::check_field(f);
+  if (!rc) rc = ki::check_field(out);
+  if (rc) return rc;
+  const int want = f->storage == KLFT_STORAGE_F64 ? KLFT_STORAGE_F64 : KLFT_STORAGE_F32;
+  if (out->storage != want || out->nd != f->nd || out->nc != f->nc)
+    return ki::set_error(KLFT_E_ARG, "staple output must be a full-storage field of the same precision and shape");
+  for (int d = 0; d < f->nd; ++d)
+    if (out->dims[d] != f->dims[d]) return ki::set_error(KLFT_E_ARG, "staple output extents differ");
+  rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    const unsigned nb = ki::blocks_for(chunk_threads<ND>(f->g), 256);
+    hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND>), dim3(nb), dim3(256), 0, ki::stream(),
+                       ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+  if (rc) return rc;
+  return ki::sync_stream();
+}
+
+// ============================================================================
+// Wilson loops (WilsonLoop.hpp:34-304)
+// ============================================================================
+// One block holds C full columns along nu (periodic), one thread per site.
+// For each requested L (ascending) a thread extends its mu-line
+// F_L(x) = U_mu(x) ... U_mu(x+(L-1)mu) and publishes it in LDS; then for
+// T = 0, 1, ... it carries the three-sided staple
+//   D(L,T) = l_nu(x;T)^+ F_L(x) l_nu(x+L mu;T),
+//   D(L,T) = U_nu(x+(T-1)nu)^+ D(L,T-1) U_nu(x+L mu+(T-1)nu),
+// and closes the loop with the top line read from LDS:
+//   w(x;L,T) = Re Tr[D(L,T) F_L(x+T nu)^+],
+// which equals the reference's Re Tr[A B^+] with A = l_mu(x;L) l_nu(x+L mu;T),
+// B = l_nu(x;T) l_mu(x+T nu;L) (WilsonLoop.hpp:129-160).  No line field is
+// materialised in HBM.  Block partial sums go to
+// partials[((plane * nL + li) * (Tmax+1) + T) * gridDim.x + blockIdx.x].
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, int mu0, int nu,
+                                                 int ncolblk, int nL, const int *__restrict__ Ls,
+                                                 int Tmax, double *partials, double *per_site,
+                                                 int ps_li, int ps_T) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Mat<R, NC> *lds = reinterpret_cast<Mat<R, NC> *>(smem);
+  __shared__ double red[8];
+  const int plane = blockIdx.y;
+  const int mu = mu0 < 0 ? plane : mu0;
+  const int Ln = g.L[nu];
+  const int tid = threadIdx.x;
+  const int cl = tid / Ln, k = tid - cl * Ln;
+  const long long ncol = g.V / Ln;
+  const long long col = (long long)blockIdx.x * ncolblk + cl;
+  const bool valid = cl < ncolblk && col < ncol;
+  int x[ND];
+  {
+    long long r = valid ? col : 0;
+#pragma unroll
+    for (int d = ND - 1; d >= 0; --d) {
+      if (d == nu) continue;
+      x[d] = (int)(r % g.L[d]);
+      r /= g.L[d];
+    }
+    x[nu] = k;
+  }
+  const int Lm = g.L[mu];
+  Mat<R, NC> F = identity<R, NC>();
+  int Lcur = 0;
+  for (int li = 0; li < nL; ++li) {
+    const int L = Ls[li];
+    if (valid) {
+      while (Lcur < L) {
+        int xs[ND];
+        copy_x<ND>(x, xs);
+        xs[mu] = shiftk(x[mu], Lcur, Lm);
+        F = mul(F, f.load(dev_index<ND>(g, xs), mu));
+        ++Lcur;
+      }
+    }
+    __syncthreads();
+    lds[tid] = F;
+    __syncthreads();
+    Mat<R, NC> D = F;
+    int xa[ND], xb[ND];
+    copy_x<ND>(x, xa);
+    copy_x<ND>(x, xb);
+    xb[mu] = shiftk(x[mu], L, Lm);
+    for (int T = 0; T <= Tmax; ++T) {
+      double w = 0.0;
+      if (valid) {
+        if (T > 0) {
+          const Mat<R, NC> a = f.load(dev_index<ND>(g, xa), nu);
+          const Mat<R, NC> b = f.load(dev_index<ND>(g, xb), nu);
+          D = mul(mul_da(a, D), b);
+          xa[nu] = up(xa[nu], Ln);
+          xb[nu] = up(xb[nu], Ln);
+        }
+        int kt = k + T;
+        kt %= Ln;
+        w = (double)retr_mul_ad(D, lds[cl * Ln + kt]);
+        if (per_site && li == ps_li && T == ps_T && plane == 0) per_site[lex_index<ND>(g, x)] = w;
+      }
+      const double bt = block_sum<512>(w, red);
+      if (tid == 0)
+        partials[((long long)(plane * nL + li) * (Tmax + 1) + T) * gridDim.x + blockIdx.x] = bt;
+    }
+  }
+}
+
+// fixed-order sum of nblk partials for each output
+__global__ __launch_bounds__(256) void k_sum_partials(const double *partials, int nblk, double *out) {
+  __shared__ double red[8];
+  const double *p = partials + (long long)blockIdx.x * nblk;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) v += p[i];
+  v = block_sum<256>(v, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+namespace {
+// sums over sites of w for the planes (mu_first.., nu); res[plane][pair]
+int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, int npairs,
+                std::vector<double> &res, double *per_site_host) {
+  if (npairs <= 0) {
+    res.clear();
+    return KLFT_OK;
+  }
+  std::vector<int> Ls;
+  int Tmax = 0;
+  for (int p = 0; p < npairs; ++p) {
+    if (pairs[2 * p] < 0 || pairs[2 * p + 1] < 0)
+      return ki::set_error(KLFT_E_ARG, "negative Wilson loop extent");
+    Ls.push_back(pairs[2 * p]);
+    Tmax = std::max(Tmax, pairs[2 * p + 1]);
+  }
+  std::sort(Ls.begin(), Ls.end());
+  Ls.erase(std::unique(Ls.begin(), Ls.end()), Ls.end());
+  const int nL = (int)Ls.size();
+  const int Ln = f->dims[nu];
+  const size_t mbytes = (size_t)f->nc * f->nc * f->cbytes;
+  int C = std::max(1, 256 / Ln);
+  // whole waves: the trailing threads (cl >= C) hold no column
+  int block = ((C * Ln + 63) / 64) * 64;
+  if (block > 512 || block * mbytes > 64 * 1024)
+    return ki::set_error(KLFT_E_ARG, "extent %d along nu too large for the Wilson-loop kernel", Ln);
+  const long long ncol = f->g.V / Ln;
+  const unsigned nbx = (unsigned)((ncol + C - 1) / C);
+  const long long nout = (long long)nplanes * nL * (Tmax + 1);
+  double *dpart = nullptr, *dout = nullptr, *dps = nullptr;
+  int *dLs = nullptr;
+  KLFT_HIP(hipMalloc(&dpart, sizeof(double) * nout * nbx));
+  KLFT_HIP(hipMalloc(&dout, sizeof(double) * nout));
+  KLFT_HIP(hipMalloc(&dLs, sizeof(int) * nL));
+  if (per_site_host) KLFT_HIP(hipMalloc(&dps, sizeof(double) * f->g.V));
+  KLFT_HIP(hipMemcpyAsync(dLs, Ls.data(), sizeof(int) * nL, hipMemcpyHostToDevice, ki::stream()));
+  int ps_li = 0, ps_T = 0;
+  if (per_site_host) {
+    const int lastL = pairs[2 * (npairs - 1)];
+    ps_li = (int)(std::lower_bound(Ls.begin(), Ls.end(), lastL) - Ls.begin());
+    ps_T = pairs[2 * (npairs - 1) + 1];
+  }
+  int rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    hipLaunchKernelGGL((k_wilson<R, NC, COMP, ND>), dim3(nbx, nplanes), dim3(block),
+                       block * sizeof(Mat<R, NC>), ki::stream(), ki::links_of<R, NC, COMP>(f), f->g,
+                       nplanes > 1 ? -1 : mu0, nu, C, nL, dLs, Tmax, dpart, dps, ps_li, ps_T);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+  if (!rc) {
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nout), dim3(256), 0, ki::stream(), dpart, (int)nbx, dout);
+    hipError_t e = hipGetLastError();
+    std::vector<double> all(nout);
+    if (e == hipSuccess) e = hipMemcpyAsync(all.data(), dout, sizeof(double) * nout, hipMemcpyDeviceToHost, ki::stream());
+    if (e == hipSuccess && per_site_host)
+      e = hipMemcpyAsync(per_site_host, dps, sizeof(double) * f->g.V, hipMemcpyDeviceToHost, ki::stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(ki::stream());
+    if (e != hipSuccess) rc = ki::hip_fail(e, "wilson loop");
+    if (!rc) {
+      res.assign((size_t)nplanes * npairs, 0.0);
+      for (int pl = 0; pl < nplanes; ++pl)
+        for (int p = 0; p < npairs; ++p) {
+          const int li = (int)(std::lower_bound(Ls.begin(), Ls.end(), pairs[2 * p]) - Ls.begin());
+          res[(size_t)pl * npairs + p] = all[((size_t)pl * nL + li) * (Tmax + 1) + pairs[2 * p + 1]];
+        }
+    }
+  }
+  (void)hipFree(dpart);
+  (void)hipFree(dout);
+  (void)hipFree(dLs);
+  if (dps) (void)hipFree(dps);
+  return rc;
+}
+
+// WilsonLoop.hpp:233-239: sequential divisions by each extent, then by Nc
+double wilson_normalize(klft_field_t f, double w, int normalize) {
+  if (normalize) {
+    for (int d = 0; d < f->nd; ++d) w /= (double)f->dims[d];
+    w /= (double)f->nc;
+  }
+  return w;
+}
+}  // namespace
+
+extern "C" int klft_wilson_loop_mu_nu(klft_field_t f, int mu, int nu, const int *pairs, int npairs,
+                                      int normalize, double *out, double *per_site) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (mu < 0 || mu >= f->nd || nu < 0 || nu >= f->nd || mu == nu)
+    return ki::set_error(KLFT_E_ARG, "invalid plane (%d, %d)", mu, nu);
+  if (npairs > 0 && (!pairs || !out)) return ki::set_error(KLFT_E_ARG, "null argument");
+  std::vector<double> res;
+  rc = wilson_sums(f, mu, 1, nu, pairs, npairs, res, per_site);
+  if (rc) return rc;
+  for (int p = 0; p < npairs; ++p) out[p] = wilson_normalize(f, res[p], normalize);
+  return KLFT_OK;
+}
+
+extern "C" int klft_wilson_loop_temporal(klft_field_t f, const int *pairs, int npairs, int normalize,
+                                         double *out) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (npairs > 0 && (!pairs || !out)) return ki::set_error(KLFT_E_ARG, "null argument");
+  const int nu = f->nd - 1;
+  std::vector<double> res;
+  rc = wilson_sums(f, 0, f->nd - 1, nu, pairs, npairs, res, nullptr);
+  if (rc) return rc;
+  // WilsonLoop.hpp:257-303: per-plane normalised values summed in mu order,
+  // then divided by (Nd - 1)
+  for (int p = 0; p < npairs; ++p) {
+    double w = wilson_normalize(f, res[p], normalize);
+    for (int pl = 1; pl < f->nd - 1; ++pl) w += wilson_normalize(f, res[(size_t)pl * npairs + p], normalize);
+    out[p] = w / (double)(f->nd - 1);
+  }
+  return KLFT_OK;
+}

@@ -1,0 +1,708 @@
+// Device-side building blocks of the klft MI355X path: complex and SU(N)
+// algebra, the sublattice-major structure-of-arrays link layout, site
+// navigation, Metropolis proposals, reunitarisation, the Lie-algebra
+// exponential and block / grid reductions.
+//
+// Link layout in HBM (one allocation per field):
+//   links[(mu * NCOMP + c) * V + s]  with  s = sub * Vh + r
+//   sub = sum_d (x_d & 1) << (nd - 1 - d)   -- the sublattice (pass) index of
+//         the reference's oddeven_array (IndexHelper.hpp:84-93)
+//   r   = lexicographic index of y_d = x_d >> 1 over the half extents, last
+//         dimension fastest (index_odd_even, IndexHelper.hpp:68-81)
+//   c   = row-major colour component; NCOMP = Nc*Nc, or (Nc-1)*Nc for the
+//         compressed two-row (SU(3)) / one-row (SU(2)) storage.
+// Each Metropolis pass therefore reads and writes whole contiguous
+// (sublattice, direction) groups, and every load is a 64-lane coalesced
+// 8- or 16-byte access.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.hpp"
+
+namespace kd {
+
+// ---------------------------------------------------------------------------
+// complex numbers (operation order of Kokkos::complex)
+// ---------------------------------------------------------------------------
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+  R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> mk(R a, R b) {
+  cx<R> c;
+  c.re = a;
+  c.im = b;
+  return c;
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+  return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) {
+  return mk(a.re + b.re, a.im + b.im);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) {
+  return mk(a.re - b.re, a.im - b.im);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> conj(cx<R> a) {
+  return mk(a.re, -a.im);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> scale(R s, cx<R> a) {
+  return mk(s * a.re, s * a.im);
+}
+// a * conj(b), the conjugate folded into the product
+template <typename R>
+__device__ __forceinline__ cx<R> mulc(cx<R> a, cx<R> b) {
+  return mk(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
+}
+// conj(a) * conj(b)
+template <typename R>
+__device__ __forceinline__ cx<R> mulcc(cx<R> a, cx<R> b) {
+  return mk(a.re * b.re - a.im * b.im, -(a.re * b.im + a.im * b.re));
+}
+// conj(a) * b
+template <typename R>
+__device__ __forceinline__ cx<R> cmul_(cx<R> a, cx<R> b) {
+  return mk(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);
+}
+
+// ---------------------------------------------------------------------------
+// Nc x Nc matrices, row major (GLOBAL.hpp:51-52)
+// ---------------------------------------------------------------------------
+template <typename R, int N>
+struct Mat {
+  cx<R> m[N][N];
+};
+
+template <typename R, int N>
+__device__ __forceinline__ Mat<R, N> zero() {
+  Mat<R, N> z;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) z.m[i][j] = mk<R>(0, 0);
+  return z;
+}
+template <typename R, int N>
+__device__ __forceinline__ Mat<R, N> identity() {
+  Mat<R, N> z = zero<R, N>();
+#pragma unroll
+  for (int i = 0; i < N; ++i) z.m[i][i].re = R(1);
+  return z;
+}
+
+// a * b, k accumulated in order (SUN.hpp:28-44)
+template <typename R, int N>
+__device__ __forceinline__ Mat<R, N> mul(const Mat<R, N>& a, const Mat<R, N>& b) {
+  Mat<R, N> c;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> t = a.m[i][0] * b.m[0][j];
+#pragma unroll
+      for (int k = 1; k < N; ++k) t = t + a.m[i][k] * b.m[k][j];
+      c.m[i][j] = t;
+    }
+  return c;
+}
+// a * b^dagger
+template <typename R, int N>
+__device__ __forceinline__ Mat<R, N> mul_ad(const Mat<R, N>& a, const Mat<R, N>& b) {
+  Mat<R, N> c;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> t = mulc(a.m[i][0], b.m[j][0]);
+#pragma unroll
+      for (int k = 1; k < N; ++k) t = t + mulc(a.m[i][k], b.m[j][k]);
+      c.m[i][j] = t;
+    }
+  return c;
+}
+// a^dagger * b
+template <typename R, int N>
+__device__ __forceinline__ Mat<R, N> mul_da(const Mat<R, N>& a, const Mat<R, N>& b) {
+  Mat<R, N> c;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> t = cmul_(a.m[0][i], b.m[0][j]);
+#pragma unroll
+      for (int k = 1; k < N; ++k) t = t + cmul_(a.m[k][i], b.m[k][j]);
+      c.m[i][j] = t;
+    }
+  return c;
+}
+// a^dagger * b^dagger
+template <typename R, int N>
+__device__ __forceinline__ Mat<R, N> mul_dd(const Mat<R, N>& a, const Mat<R, N>& b) {
+  Mat<R, N> c;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> t = mulcc(a.m[0][i], b.m[j][0]);
+#pragma unroll
+      for (int k = 1; k < N; ++k) t = t + mulcc(a.m[k][i], b.m[j][k]);
+      c.m[i][j] = t;
+    }
+  return c;
+}
+template <typename R, int N>
+__device__ __forceinline__ void add_to(Mat<R, N>& a, const Mat<R, N>& b) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) a.m[i][j] = a.m[i][j] + b.m[i][j];
+}
+// Re Tr(a * b)
+template <typename R, int N>
+__device__ __forceinline__ R retr_mul(const Mat<R, N>& a, const Mat<R, N>& b) {
+  R t = R(0);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int k = 0; k < N; ++k) t += a.m[i][k].re * b.m[k][i].re - a.m[i][k].im * b.m[k][i].im;
+  return t;
+}
+// Re Tr(a * b^dagger) = sum_ij Re(a_ij conj(b_ij)), summed c1 outer, c2 inner
+template <typename R, int N>
+__device__ __forceinline__ R retr_mul_ad(const Mat<R, N>& a, const Mat<R, N>& b) {
+  R t = R(0);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) t += a.m[i][j].re * b.m[i][j].re + a.m[i][j].im * b.m[i][j].im;
+  return t;
+}
+template <typename R, int N>
+__device__ __forceinline__ R retrace(const Mat<R, N>& a) {
+  R t = R(0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) t += a.m[i][i].re;
+  return t;
+}
+
+template <typename RO, typename RI, int N>
+__device__ __forceinline__ Mat<RO, N> convert(const Mat<RI, N>& a) {
+  Mat<RO, N> c;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) c.m[i][j] = mk<RO>((RO)a.m[i][j].re, (RO)a.m[i][j].im);
+  return c;
+}
+
+// third row of an SU(3) matrix from the first two: conj(r0 x r1), the
+// reference's own completion (SUN.hpp:334-336)
+template <typename R>
+__device__ __forceinline__ void complete_row2(Mat<R, 3>& a) {
+  a.m[2][0] = conj(a.m[0][1] * a.m[1][2] - a.m[0][2] * a.m[1][1]);
+  a.m[2][1] = conj(a.m[0][2] * a.m[1][0] - a.m[0][0] * a.m[1][2]);
+  a.m[2][2] = conj(a.m[0][0] * a.m[1][1] - a.m[0][1] * a.m[1][0]);
+}
+template <typename R>
+__device__ __forceinline__ void complete_row1(Mat<R, 2>& a) {
+  a.m[1][0] = mk(-a.m[0][1].re, a.m[0][1].im);
+  a.m[1][1] = mk(a.m[0][0].re, -a.m[0][0].im);
+}
+
+// ---------------------------------------------------------------------------
+// reunitarisation
+// ---------------------------------------------------------------------------
+// restoreSUN<1,2,3> exactly as the reference (SUN.hpp:256-340): the plain FP64
+// method uses this one.
+template <typename R>
+__device__ __forceinline__ void restore_sun(Mat<R, 1>& a) {
+  const R n = sqrt(a.m[0][0].re * a.m[0][0].re + a.m[0][0].im * a.m[0][0].im);
+  a.m[0][0] = mk(a.m[0][0].re / n, a.m[0][0].im / n);
+}
+template <typename R>
+__device__ __forceinline__ void restore_sun(Mat<R, 2>& a) {
+  const R n = sqrt(a.m[0][0].re * a.m[0][0].re + a.m[0][0].im * a.m[0][0].im +
+                   a.m[0][1].re * a.m[0][1].re + a.m[0][1].im * a.m[0][1].im);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) a.m[i][j] = mk(a.m[i][j].re / n, a.m[i][j].im / n);
+}
+template <typename R>
+__device__ __forceinline__ void restore_sun(Mat<R, 3>& a) {
+  const cx<R> s0 = cmul_(a.m[0][0], a.m[0][0]) + cmul_(a.m[0][1], a.m[0][1]) +
+                   cmul_(a.m[0][2], a.m[0][2]);
+  const cx<R> s1 = cmul_(a.m[1][0], a.m[1][0]) + cmul_(a.m[1][1], a.m[1][1]) +
+                   cmul_(a.m[1][2], a.m[1][2]);
+  const R n0 = sqrt(s0.re), n1 = sqrt(s1.re);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    a.m[0][j] = mk(a.m[0][j].re / n0, a.m[0][j].im / n0);
+    a.m[1][j] = mk(a.m[1][j].re / n1, a.m[1][j].im / n1);
+  }
+  complete_row2(a);
+  a.m[1][0] = conj(a.m[2][1] * a.m[0][2] - a.m[2][2] * a.m[0][1]);
+  a.m[1][1] = conj(a.m[2][2] * a.m[0][0] - a.m[2][0] * a.m[0][2]);
+  a.m[1][2] = conj(a.m[2][0] * a.m[0][1] - a.m[2][1] * a.m[0][0]);
+}
+
+// Gram-Schmidt reunitarisation for FP32 links: products in FP32, every norm
+// accumulated and square-rooted in FP64.
+__device__ __forceinline__ float inv_norm64(double n2) { return (float)(1.0 / sqrt(n2)); }
+
+__device__ __forceinline__ void reunit_gs(Mat<float, 1>& a) {
+  const double n2 = (double)a.m[0][0].re * a.m[0][0].re + (double)a.m[0][0].im * a.m[0][0].im;
+  const float f = inv_norm64(n2);
+  a.m[0][0] = scale(f, a.m[0][0]);
+}
+__device__ __forceinline__ void reunit_gs(Mat<float, 2>& a) {
+  double n2 = 0.0;
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+    n2 += (double)a.m[0][j].re * a.m[0][j].re + (double)a.m[0][j].im * a.m[0][j].im;
+  const float f = inv_norm64(n2);
+  a.m[0][0] = scale(f, a.m[0][0]);
+  a.m[0][1] = scale(f, a.m[0][1]);
+  complete_row1(a);
+}
+__device__ __forceinline__ void reunit_gs(Mat<float, 3>& a) {
+  double n2 = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    n2 += (double)a.m[0][j].re * a.m[0][j].re + (double)a.m[0][j].im * a.m[0][j].im;
+  float f = inv_norm64(n2);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) a.m[0][j] = scale(f, a.m[0][j]);
+  // r1 -= (r0^dagger r1) r0
+  cx<float> p = cmul_(a.m[0][0], a.m[1][0]) + cmul_(a.m[0][1], a.m[1][1]) +
+                cmul_(a.m[0][2], a.m[1][2]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) a.m[1][j] = a.m[1][j] - p * a.m[0][j];
+  n2 = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    n2 += (double)a.m[1][j].re * a.m[1][j].re + (double)a.m[1][j].im * a.m[1][j].im;
+  f = inv_norm64(n2);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) a.m[1][j] = scale(f, a.m[1][j]);
+  complete_row2(a);
+}
+
+// policy: FP64 links are restored with the reference's restoreSUN; FP32 links
+// with the FP64-norm Gram-Schmidt above.
+template <int N>
+__device__ __forceinline__ void reunitarize(Mat<double, N>& a) {
+  restore_sun(a);
+}
+template <int N>
+__device__ __forceinline__ void reunitarize(Mat<float, N>& a) {
+  reunit_gs(a);
+}
+
+// ---------------------------------------------------------------------------
+// lattice geometry
+// ---------------------------------------------------------------------------
+struct Geom {
+  int nd;
+  int L[4];
+  int H[4];          // L / 2
+  long long V, Vh;   // volume, volume / 2^nd
+  long long hs[4];   // strides of the half-extent (y) lexicographic index
+  long long ls[4];   // strides of the full lexicographic index
+};
+
+template <int ND>
+__device__ __forceinline__ long long dev_index(const Geom& g, const int* x) {
+  int sub = 0;
+  long long r = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    sub |= (x[d] & 1) << (ND - 1 - d);
+    r += (long long)(x[d] >> 1) * g.hs[d];
+  }
+  return (long long)sub * g.Vh + r;
+}
+template <int ND>
+__device__ __forceinline__ long long lex_index(const Geom& g, const int* x) {
+  long long r = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) r += (long long)x[d] * g.ls[d];
+  return r;
+}
+// device index -> coordinates
+template <int ND>
+__device__ __forceinline__ void dev_coords(const Geom& g, long long s, int* x) {
+  const int sub = (int)(s / g.Vh);
+  long long r = s - (long long)sub * g.Vh;
+#pragma unroll
+  for (int d = ND - 1; d >= 0; --d) {
+    const int y = (int)(r % g.H[d]);
+    r /= g.H[d];
+    x[d] = 2 * y + ((sub >> (ND - 1 - d)) & 1);
+  }
+}
+// lexicographic index -> coordinates
+template <int ND>
+__device__ __forceinline__ void lex_coords(const Geom& g, long long s, int* x) {
+#pragma unroll
+  for (int d = ND - 1; d >= 0; --d) {
+    x[d] = (int)(s % g.L[d]);
+    s /= g.L[d];
+  }
+}
+template <int ND>
+__device__ __forceinline__ void copy_x(const int* a, int* b) {
+#pragma unroll
+  for (int d = 0; d < ND; ++d) b[d] = a[d];
+}
+// periodic shift of one coordinate by +-1 (shift_index_plus/minus)
+__device__ __forceinline__ int up(int v, int L) { return v + 1 == L ? 0 : v + 1; }
+__device__ __forceinline__ int dn(int v, int L) { return v == 0 ? L - 1 : v - 1; }
+__device__ __forceinline__ int shiftk(int v, int k, int L) {
+  int r = (v + k) % L;
+  return r < 0 ? r + L : r;
+}
+
+// ---------------------------------------------------------------------------
+// link field access
+// ---------------------------------------------------------------------------
+template <typename R, int NC, bool COMP>
+struct Links {
+  static constexpr bool C = COMP && NC > 1;
+  static constexpr int NCOMP = C ? (NC - 1) * NC : NC * NC;
+  cx<R>* __restrict__ p;
+  long long V;
+
+  __device__ __forceinline__ Mat<R, NC> load(long long s, int mu) const {
+    Mat<R, NC> u;
+    const cx<R>* q = p + (long long)mu * NCOMP * V + s;
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) u.m[c / NC][c % NC] = q[(long long)c * V];
+    if constexpr (C) {
+      if constexpr (NC == 3) complete_row2(u);
+      if constexpr (NC == 2) complete_row1(u);
+    }
+    return u;
+  }
+  __device__ __forceinline__ void store(long long s, int mu, const Mat<R, NC>& u) const {
+    cx<R>* q = p + (long long)mu * NCOMP * V + s;
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) q[(long long)c * V] = u.m[c / NC][c % NC];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// staple, GaugeField.hpp:187-237 (3D :389-433, 2D :579-618):
+//   S = sum_{nu != mu} U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+//     + sum_{nu != mu} U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+// products associated left to right, the upper staples first.
+// ---------------------------------------------------------------------------
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ Mat<R, NC> staple(const Links<R, NC, COMP>& f, const Geom& g,
+                                             const int* x, int mu) {
+  Mat<R, NC> t = zero<R, NC>();
+  int xp[ND];
+  copy_x<ND>(x, xp);
+  xp[mu] = up(x[mu], g.L[mu]);
+  const long long s_x = dev_index<ND>(g, x);
+  const long long s_xpmu = dev_index<ND>(g, xp);
+#pragma unroll
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == mu) continue;
+    int xn[ND];
+    copy_x<ND>(x, xn);
+    xn[nu] = up(x[nu], g.L[nu]);
+    const Mat<R, NC> a = f.load(s_xpmu, nu);
+    const Mat<R, NC> b = f.load(dev_index<ND>(g, xn), mu);
+    const Mat<R, NC> c = f.load(s_x, nu);
+    add_to(t, mul_ad(mul_ad(a, b), c));
+  }
+#pragma unroll
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == mu) continue;
+    int xa[ND], xb[ND];
+    copy_x<ND>(xp, xa);
+    xa[nu] = dn(xp[nu], g.L[nu]);
+    copy_x<ND>(x, xb);
+    xb[nu] = dn(x[nu], g.L[nu]);
+    const long long s_xmnu = dev_index<ND>(g, xb);
+    const Mat<R, NC> a = f.load(dev_index<ND>(g, xa), nu);
+    const Mat<R, NC> b = f.load(s_xmnu, mu);
+    const Mat<R, NC> c = f.load(s_xmnu, nu);
+    add_to(t, mul(mul_dd(a, b), c));
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Metropolis proposals, randSUN<1,2,3> (SUN.hpp:163-251), drawn from the
+// keyed stream in the reference's order.  FP64 draws take 53-bit doubles;
+// FP32 draws take 24-bit floats (half the generator work).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double draw(klft_rng::Stream& s, double lo, double hi) {
+  return lo + (hi - lo) * s.u01();
+}
+__device__ __forceinline__ float draw(klft_rng::Stream& s, float lo, float hi) {
+  return lo + (hi - lo) * s.u01f();
+}
+template <typename R>
+__device__ __forceinline__ R unif(klft_rng::Stream& s);
+template <>
+__device__ __forceinline__ double unif<double>(klft_rng::Stream& s) {
+  return s.u01();
+}
+template <>
+__device__ __forceinline__ float unif<float>(klft_rng::Stream& s) {
+  return s.u01f();
+}
+
+#define KLFT_PI 3.14159265358979323846
+
+template <typename R>
+__device__ __forceinline__ void rand_sun(klft_rng::Stream& s, R delta, Mat<R, 1>& r) {
+  const R pi = (R)KLFT_PI;
+  const R th = draw(s, -delta * pi, delta * pi);
+  r.m[0][0] = mk(cos(th), sin(th));
+}
+template <typename R>
+__device__ __forceinline__ void rand_sun(klft_rng::Stream& s, R delta, Mat<R, 2>& r) {
+  const R pi = (R)KLFT_PI;
+  const R alpha = draw(s, R(0), delta * 2 * pi);
+  const R u = draw(s, R(-1), R(1));
+  const R theta = draw(s, R(0), R(2) * pi);
+  const R sa = sin(alpha);
+  const R n1 = sqrt(R(1) - u * u) * sin(theta);
+  const R n2 = sqrt(R(1) - u * u) * cos(theta);
+  r.m[0][0] = mk(cos(alpha), u * sa);
+  r.m[0][1] = mk(n1 * sa, n2 * sa);
+  r.m[1][0] = mk(-r.m[0][1].re, r.m[0][1].im);
+  r.m[1][1] = mk(r.m[0][0].re, -r.m[0][0].im);
+}
+template <typename R>
+__device__ __forceinline__ void rand_sun(klft_rng::Stream& s, R delta, Mat<R, 3>& r) {
+  R v[6], norm, fact;
+  cx<R> z1[3], z2[3];
+  for (;;) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v[i] = draw(s, R(0), delta);
+    norm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+    if (R(1) != (R(1) + norm)) break;
+  }
+  fact = R(1) / norm;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) z1[i] = mk(fact * v[2 * i], fact * v[2 * i + 1]);
+  for (;;) {
+    for (;;) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) v[i] = draw(s, R(0), delta);
+      norm =
+          sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+      if (R(1) != (R(1) + norm)) break;
+    }
+    fact = R(1) / norm;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) z2[i] = mk(fact * v[2 * i], fact * v[2 * i + 1]);
+    const cx<R> z = cmul_(z1[0], z2[0]) + cmul_(z1[1], z2[1]) + cmul_(z1[2], z2[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) z2[i] = z2[i] - z * z1[i];
+    norm = sqrt(z2[0].re * z2[0].re + z2[0].im * z2[0].im + z2[1].re * z2[1].re +
+                z2[1].im * z2[1].im + z2[2].re * z2[2].re + z2[2].im * z2[2].im);
+    if (R(1) != (R(1) + norm)) break;
+  }
+  fact = R(1) / norm;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    z2[i] = scale(fact, z2[i]);
+    r.m[0][i] = z1[i];
+    r.m[1][i] = z2[i];
+  }
+  complete_row2(r);
+}
+
+// ---------------------------------------------------------------------------
+// Lie algebra: Q = sum_a v_a T_a with T = 1 (U(1)), Pauli sigma (SU(2)),
+// Gell-Mann lambda (SU(3), Tr T_a T_b = 2 delta_ab).
+// ---------------------------------------------------------------------------
+template <int NC>
+struct Ngen;
+template <>
+struct Ngen<1> {
+  static constexpr int value = 1;
+};
+template <>
+struct Ngen<2> {
+  static constexpr int value = 3;
+};
+template <>
+struct Ngen<3> {
+  static constexpr int value = 8;
+};
+
+// F_a = c * Im Tr(T_a M)  (AdjointGroup.hpp:13-19 for SU(2), :114-117 for U(1))
+template <typename R>
+__device__ __forceinline__ void project(const Mat<R, 1>& M, R c, R* F) {
+  F[0] = c * M.m[0][0].im;
+}
+template <typename R>
+__device__ __forceinline__ void project(const Mat<R, 2>& M, R c, R* F) {
+  F[0] = c * (M.m[0][1].im + M.m[1][0].im);
+  F[1] = c * (M.m[0][1].re - M.m[1][0].re);
+  F[2] = c * (M.m[0][0].im - M.m[1][1].im);
+}
+template <typename R>
+__device__ __forceinline__ void project(const Mat<R, 3>& M, R c, R* F) {
+  const R r3 = (R)0.57735026918962576451;
+  F[0] = c * (M.m[0][1].im + M.m[1][0].im);
+  F[1] = c * (M.m[0][1].re - M.m[1][0].re);
+  F[2] = c * (M.m[0][0].im - M.m[1][1].im);
+  F[3] = c * (M.m[0][2].im + M.m[2][0].im);
+  F[4] = c * (M.m[0][2].re - M.m[2][0].re);
+  F[5] = c * (M.m[1][2].im + M.m[2][1].im);
+  F[6] = c * (M.m[1][2].re - M.m[2][1].re);
+  F[7] = c * ((M.m[0][0].im + M.m[1][1].im - R(2) * M.m[2][2].im) * r3);
+}
+
+// exp(i v) for U(1) (AdjointGroup.hpp:170-173)
+template <typename R>
+__device__ __forceinline__ Mat<R, 1> exp_iq(const R* v, Mat<R, 1>*) {
+  Mat<R, 1> e;
+  e.m[0][0] = mk(cos(v[0]), sin(v[0]));
+  return e;
+}
+// exp(i v.sigma) for SU(2) (AdjointGroup.hpp:98-105); alpha = 0 gives the
+// identity instead of the reference's division by zero.
+template <typename R>
+__device__ __forceinline__ Mat<R, 2> exp_iq(const R* v, Mat<R, 2>*) {
+  const R alpha = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (alpha == R(0)) return identity<R, 2>();
+  const R n0 = v[0] / alpha, n1 = v[1] / alpha, n2 = v[2] / alpha;
+  const R sa = sin(alpha), ca = cos(alpha);
+  Mat<R, 2> e;
+  e.m[0][0] = mk(ca, n2 * sa);
+  e.m[0][1] = mk(n1 * sa, n0 * sa);
+  e.m[1][0] = mk(-n1 * sa, n0 * sa);
+  e.m[1][1] = mk(ca, -n2 * sa);
+  return e;
+}
+// exp(i Q) for SU(3) by the Cayley-Hamilton reduction of the exponential
+// series (Morningstar & Peardon, Phys. Rev. D 69, 054501 (2004), sec. III):
+// Q^3 = c1 Q + c0 I with c0 = det Q, c1 = Tr Q^2 / 2, so
+// exp(iQ) = f0 I + f1 Q + f2 Q^2, and the coefficients of Q^n in the basis
+// {I, Q, Q^2} follow a_{n+1} = c0 c_n, b_{n+1} = a_n + c1 c_n, c_{n+1} = b_n.
+// The series is summed until the next term is below the working precision.
+template <typename R>
+__device__ __forceinline__ Mat<R, 3> exp_iq(const R* v, Mat<R, 3>*) {
+  const R r3 = (R)0.57735026918962576451;
+  Mat<R, 3> q;
+  q.m[0][0] = mk(v[2] + v[7] * r3, R(0));
+  q.m[1][1] = mk(-v[2] + v[7] * r3, R(0));
+  q.m[2][2] = mk(R(-2) * v[7] * r3, R(0));
+  q.m[0][1] = mk(v[0], -v[1]);
+  q.m[1][0] = mk(v[0], v[1]);
+  q.m[0][2] = mk(v[3], -v[4]);
+  q.m[2][0] = mk(v[3], v[4]);
+  q.m[1][2] = mk(v[5], -v[6]);
+  q.m[2][1] = mk(v[5], v[6]);
+  const Mat<R, 3> q2 = mul(q, q);
+  const R c1 = R(0.5) * retrace(q2);
+  const Mat<R, 3> q3 = mul(q2, q);
+  const R c0 = retrace(q3) / R(3);
+  // a, b, c: coefficients of Q^n / n! ; f accumulates i^n Q^n / n!
+  R a = 1, b = 0, c = 0;
+  cx<R> f0 = mk<R>(1, 0), f1 = mk<R>(0, 0), f2 = mk<R>(0, 0);
+  const R eps = sizeof(R) == 8 ? (R)1e-18 : (R)1e-9;
+  // spectral radius bound: sum of squared eigenvalues is 2 c1
+  const R rho = sqrt(R(2) * c1);
+  for (int n = 1; n < 64; ++n) {
+    const R inv = R(1) / (R)n;
+    const R an = c0 * c * inv, bn = (a + c1 * c) * inv, cn = b * inv;
+    a = an;
+    b = bn;
+    c = cn;
+    // i^n
+    switch (n & 3) {
+      case 0: f0.re += a; f1.re += b; f2.re += c; break;
+      case 1: f0.im += a; f1.im += b; f2.im += c; break;
+      case 2: f0.re -= a; f1.re -= b; f2.re -= c; break;
+      default: f0.im -= a; f1.im -= b; f2.im -= c; break;
+    }
+    if (n > 3 && fabs(a) + (fabs(b) + fabs(c) * rho) * rho < eps) break;
+  }
+  Mat<R, 3> e;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cx<R> t = f1 * q.m[i][j] + f2 * q2.m[i][j];
+      if (i == j) t = t + f0;
+      e.m[i][j] = t;
+    }
+  return e;
+}
+
+// ---------------------------------------------------------------------------
+// reductions (wave64 shuffles -> LDS -> per-block partial)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// sum over the block in a fixed order; the result is valid in thread 0
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (int)((blockDim.x + 63) / 64);
+#pragma unroll
+    for (int i = 0; i < BLOCK / 64; ++i)
+      if (i < nw) t += lds[i];
+  }
+  __syncthreads();
+  return t;
+}
+
+// Single-pass grid reduction: every block writes its partial, the last block
+// to arrive (agent-scope ticket) sums the partials in block order and resets
+// the ticket, so the result is independent of scheduling.
+template <int BLOCK>
+__device__ __forceinline__ void grid_finish(double block_total, double* partials,
+                                            unsigned int* ticket, double* out, double* lds) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = block_total;
+    const unsigned int t =
+        __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  double v = 0.0;
+  for (unsigned int i = threadIdx.x; i < gridDim.x; i += BLOCK)
+    v += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  v = block_sum<BLOCK>(v, lds);
+  if (threadIdx.x == 0) {
+    *out = v;
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+}  // namespace kd

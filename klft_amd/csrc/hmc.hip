@@ -1,0 +1,421 @@
+// HMC for the Wilson gauge action: momentum refresh, fused force + momentum
+// update, link exponential update, kinetic / gauge action sums and the
+// leapfrog trajectory with FP64 accept/reject.
+//
+// Semantics follow the reference's HMC stack (HMC.hpp:48-71,
+// Integrator.hpp:26-53, GaugeMonomial.hpp:19-37, HamiltonianField.hpp:23-60,
+// AdjointGroup.hpp), generalised to one formula for U(1), SU(2), SU(3):
+//   P = sum_a p_a T_a, p_a ~ N(0,1);  H_kin = 1/2 sum p_a^2;
+//   H_g = -(beta/Nc) sum_x sum_{mu<nu} Re Tr U_munu(x);
+//   F_a(x,mu) = (beta/Nc) Im Tr(T_a U_mu(x) Sigma_mu(x));
+//   P <- P - dtau F;  U <- exp(i dtau P) U.
+// Momenta are stored structure-of-arrays: P[(mu * NGEN + a) * V + s] in the
+// link precision, s the sublattice-major device index.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace kd;
+namespace ki = klft_internal;
+
+struct klft_hmc_s {
+  klft_field_t f = nullptr;
+  klft_hmc_params p{};
+  void *P = nullptr;       // momenta (R)
+  void *Ubak = nullptr;    // saved links for a rejected trajectory
+  double *Fdbg = nullptr;  // force in host (AoS) order, for klft_hmc_force
+};
+
+template <typename R, int NC, int ND>
+__global__ __launch_bounds__(256) void k_refresh(R *P, Geom g, uint64_t seed, uint32_t traj,
+                                                  double *partials, unsigned int *ticket,
+                                                  double *out) {
+  __shared__ double lds[8];
+  constexpr int NG = Ngen<NC>::value;
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;  // device order
+  double k = 0.0;
+  if (s < g.V) {
+    int x[ND];
+    dev_coords<ND>(g, s, x);
+    const uint32_t site = (uint32_t)lex_index<ND>(g, x);
+#pragma unroll
+    for (int mu = 0; mu < ND; ++mu) {
+      klft_rng::Stream st(seed, klft_rng::TAG_MOMENTA, site, traj, 0, 0, (uint32_t)mu);
+      double z[NG + 1];
+#pragma unroll
+      for (int a = 0; a < NG; a += 2) {
+        const double u1 = 1.0 - st.u01(), u2 = st.u01();
+        const double r = sqrt(-2.0 * log(u1)), t = 2.0 * KLFT_PI * u2;
+        z[a] = r * cos(t);
+        z[a + 1] = r * sin(t);
+      }
+      double n2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < NG; ++a) {
+        const R v = (R)z[a];
+        P[((long long)mu * NG + a) * g.V + s] = v;
+        n2 += (double)v * (double)v;
+      }
+      k += 0.5 * n2;
+    }
+  }
+  const double bt = block_sum<256>(k, lds);
+  grid_finish<256>(bt, partials, ticket, out, lds);
+}
+
+// P -= dtau * F with F recomputed from the current links.  Fout (optional):
+// write F in host AoS order instead of updating P.  kin (optional): also
+// reduce 1/2 sum p^2 of the updated momenta.
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_force(Links<R, NC, COMP> f, R *P, Geom g, R coef, R dtau,
+                                                double *Fout, double *partials,
+                                                unsigned int *ticket, double *kin) {
+  __shared__ double lds[8];
+  constexpr int NG = Ngen<NC>::value;
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  int x[ND];
+  long long s;
+  double k = 0.0;
+  if (chunk_site<ND>(g, lb, x, &s)) {
+#pragma unroll 1
+    for (int mu = 0; mu < ND; ++mu) {
+      const Mat<R, NC> S = staple<R, NC, COMP, ND>(f, g, x, mu);
+      const Mat<R, NC> M = mul(f.load(s, mu), S);
+      R F[NG];
+      project(M, coef, F);
+      if (Fout) {
+        const long long site = lex_index<ND>(g, x);
+#pragma unroll
+        for (int a = 0; a < NG; ++a) Fout[(site * ND + mu) * NG + a] = (double)F[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < NG; ++a) {
+          R *q = P + ((long long)mu * NG + a) * g.V + s;
+          const R v = *q - dtau * F[a];
+          *q = v;
+          k += 0.5 * ((double)v * (double)v);
+        }
+      }
+    }
+  }
+  if (kin) {
+    const double bt = block_sum<256>(k, lds);
+    grid_finish<256>(bt, partials, ticket, kin, lds);
+  }
+}
+
+// U <- exp(i dtau P) U; FP32 links are reunitarised after the update
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_update_gauge(Links<R, NC, COMP> f, const R *__restrict__ P,
+                                                       Geom g, R dtau) {
+  constexpr int NG = Ngen<NC>::value;
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= g.V) return;
+#pragma unroll 1
+  for (int mu = 0; mu < ND; ++mu) {
+    R v[NG];
+#pragma unroll
+    for (int a = 0; a < NG; ++a) v[a] = dtau * P[((long long)mu * NG + a) * g.V + s];
+    const Mat<R, NC> e = exp_iq(v, (Mat<R, NC> *)nullptr);
+    Mat<R, NC> u = mul(e, f.load(s, mu));
+    if constexpr (sizeof(R) == 4) reunitarize(u);
+    f.store(s, mu, u);
+  }
+}
+
+template <typename R, int NG>
+__global__ __launch_bounds__(256) void k_kinetic_soa(const R *__restrict__ P, long long V, int nd,
+                                                      double *partials, unsigned int *ticket,
+                                                      double *out) {
+  __shared__ double lds[8];
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  double k = 0.0;
+  if (s < V) {
+    for (int mu = 0; mu < nd; ++mu) {
+      double n2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < NG; ++a) {
+        const double v = (double)P[((long long)mu * NG + a) * V + s];
+        n2 += v * v;
+      }
+      k += 0.5 * n2;
+    }
+  }
+  const double bt = block_sum<256>(k, lds);
+  grid_finish<256>(bt, partials, ticket, out, lds);
+}
+
+template <typename R, int NG, int ND>
+__global__ __launch_bounds__(256) void k_mom_io(R *P, Geom g, double *host_img, int upload) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;  // lexicographic
+  if (s >= g.V) return;
+  int x[ND];
+  lex_coords<ND>(g, s, x);
+  const long long d = dev_index<ND>(g, x);
+  for (int mu = 0; mu < ND; ++mu)
+#pragma unroll
+    for (int a = 0; a < NG; ++a) {
+      R *q = P + ((long long)mu * NG + a) * g.V + d;
+      double *h = host_img + ((size_t)s * ND + mu) * NG + a;
+      if (upload)
+        *q = (R)*h;
+      else
+        *h = (double)*q;
+    }
+}
+
+namespace {
+
+int ngen(int nc) { return nc == 1 ? 1 : (nc == 2 ? 3 : 8); }
+
+int check_hmc(klft_hmc_t h) {
+  if (!h || !h->P) return ki::set_error(KLFT_E_ARG, "null or destroyed HMC handle");
+  return ki::check_field(h->f);
+}
+
+int enqueue_refresh(klft_hmc_t h, uint64_t traj, int slot) {
+  klft_field_t f = h->f;
+  return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    (void)COMP;
+    const unsigned nb = ki::blocks_for(f->g.V, 256);
+    if (nb > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+    hipLaunchKernelGGL((k_refresh<R, NC, ND>), dim3(nb), dim3(256), 0, ki::stream(), (R *)h->P, f->g,
+                       h->p.seed, (uint32_t)traj, f->partials, f->ticket, f->result + slot);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+}
+
+int enqueue_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot) {
+  klft_field_t f = h->f;
+  return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    const unsigned nb = ki::blocks_for(chunk_threads<ND>(f->g), 256);
+    if (nb > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+    hipLaunchKernelGGL((k_force<R, NC, COMP, ND>), dim3(nb), dim3(256), 0, ki::stream(),
+                       ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC),
+                       (R)dtau, Fout, f->partials, f->ticket,
+                       kin_slot >= 0 ? f->result + kin_slot : (double *)nullptr);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+}
+
+int enqueue_gauge(klft_hmc_t h, double dtau) {
+  klft_field_t f = h->f;
+  return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    hipLaunchKernelGGL((k_update_gauge<R, NC, COMP, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256),
+                       0, ki::stream(), ki::links_of<R, NC, COMP>(f), (const R *)h->P, f->g, (R)dtau);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+}
+
+int enqueue_kinetic(klft_hmc_t h, int slot) {
+  klft_field_t f = h->f;
+  return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    (void)COMP;
+    (void)ND;
+    if (ki::blocks_for(f->g.V, 256) > (unsigned)ki::kMaxPartials)
+      return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+    hipLaunchKernelGGL((k_kinetic_soa<R, Ngen<NC>::value>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256),
+                       0, ki::stream(), (const R *)h->P, f->g.V, f->nd, f->partials, f->ticket,
+                       f->result + slot);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+}
+
+// Leapfrog::integrate (Integrator.hpp:26-53): dtau = tau/n; half momentum
+// step, gauge step, n x (momentum, gauge), half momentum step, restoreGauge.
+int enqueue_leapfrog(klft_hmc_t h, int kin_slot) {
+  const double dtau = h->p.tau / (double)h->p.nsteps;
+  int rc = enqueue_force(h, dtau / 2.0, nullptr, -1);
+  if (!rc) rc = enqueue_gauge(h, dtau);
+  for (int i = 0; i < h->p.nsteps && !rc; ++i) {
+    rc = enqueue_force(h, dtau, nullptr, -1);
+    if (!rc) rc = enqueue_gauge(h, dtau);
+  }
+  if (!rc) rc = enqueue_force(h, dtau / 2.0, nullptr, kin_slot);
+  if (!rc) rc = ki::enqueue_reunit(h->f);
+  return rc;
+}
+
+int momenta_io(klft_hmc_t h, double *host, int upload) {
+  klft_field_t f = h->f;
+  const size_t n = (size_t)f->g.V * f->nd * ngen(f->nc) * sizeof(double);
+  double *stage = nullptr;
+  KLFT_HIP(hipMalloc(&stage, n));
+  hipError_t e = hipSuccess;
+  if (upload) e = hipMemcpyAsync(stage, host, n, hipMemcpyHostToDevice, ki::stream());
+  int rc = KLFT_OK;
+  if (e == hipSuccess)
+    rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+      KLFT_UNPACK(rt, nct, ct, ndt);
+      (void)COMP;
+      hipLaunchKernelGGL((k_mom_io<R, Ngen<NC>::value, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256), 0,
+                         ki::stream(), (R *)h->P, f->g, stage, upload);
+      KLFT_HIP(hipGetLastError());
+      return KLFT_OK;
+    });
+  if (e == hipSuccess && !rc && !upload) e = hipMemcpyAsync(host, stage, n, hipMemcpyDeviceToHost, ki::stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ki::stream());
+  (void)hipFree(stage);
+  if (e != hipSuccess) return ki::hip_fail(e, "momenta transfer");
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int klft_hmc_create(klft_field_t f, const klft_hmc_params *p, klft_hmc_t *out) {
+  if (!out || !p) return ki::set_error(KLFT_E_ARG, "null argument");
+  *out = nullptr;
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (p->nsteps < 1) return ki::set_error(KLFT_E_ARG, "n_steps must be >= 1");
+  if (!(p->tau > 0.0)) return ki::set_error(KLFT_E_ARG, "tau must be > 0");
+  klft_hmc_s *h = new klft_hmc_s();
+  h->f = f;
+  h->p = *p;
+  const size_t rbytes = f->storage == KLFT_STORAGE_F64 ? 8 : 4;
+  const size_t pbytes = (size_t)f->g.V * f->nd * ngen(f->nc) * rbytes;
+  hipError_t e = hipMalloc(&h->P, pbytes);
+  if (e == hipSuccess) e = hipMalloc(&h->Ubak, f->link_bytes);
+  if (e == hipSuccess) e = hipMemset(h->P, 0, pbytes);
+  if (e != hipSuccess) {
+    if (h->P) (void)hipFree(h->P);
+    if (h->Ubak) (void)hipFree(h->Ubak);
+    delete h;
+    return ki::set_error(KLFT_E_NOMEM, "HMC workspace allocation failed");
+  }
+  *out = h;
+  return KLFT_OK;
+}
+
+extern "C" int klft_hmc_destroy(klft_hmc_t h) {
+  if (!h) return KLFT_OK;
+  (void)ki::sync_stream();
+  if (h->P) (void)hipFree(h->P);
+  if (h->Ubak) (void)hipFree(h->Ubak);
+  if (h->Fdbg) (void)hipFree(h->Fdbg);
+  h->P = nullptr;
+  delete h;
+  return KLFT_OK;
+}
+
+extern "C" int klft_hmc_refresh_momenta(klft_hmc_t h, uint64_t traj) {
+  int rc = check_hmc(h);
+  if (!rc) rc = enqueue_refresh(h, traj, 1);
+  if (rc) return rc;
+  return ki::sync_stream();
+}
+
+extern "C" int klft_hmc_momenta_upload(klft_hmc_t h, const double *host) {
+  int rc = check_hmc(h);
+  if (rc) return rc;
+  if (!host) return ki::set_error(KLFT_E_ARG, "null host buffer");
+  return momenta_io(h, const_cast<double *>(host), 1);
+}
+
+extern "C" int klft_hmc_momenta_download(klft_hmc_t h, double *host) {
+  int rc = check_hmc(h);
+  if (rc) return rc;
+  if (!host) return ki::set_error(KLFT_E_ARG, "null host buffer");
+  return momenta_io(h, host, 0);
+}
+
+extern "C" int klft_hmc_force(klft_hmc_t h, double *host_force) {
+  int rc = check_hmc(h);
+  if (rc) return rc;
+  if (!host_force) return ki::set_error(KLFT_E_ARG, "null host buffer");
+  klft_field_t f = h->f;
+  const size_t n = (size_t)f->g.V * f->nd * ngen(f->nc);
+  if (!h->Fdbg) KLFT_HIP(hipMalloc(&h->Fdbg, n * sizeof(double)));
+  rc = enqueue_force(h, 0.0, h->Fdbg, -1);
+  if (rc) return rc;
+  KLFT_HIP(hipMemcpyAsync(host_force, h->Fdbg, n * sizeof(double), hipMemcpyDeviceToHost, ki::stream()));
+  return ki::sync_stream();
+}
+
+extern "C" int klft_hmc_update_momentum(klft_hmc_t h, double dtau) {
+  int rc = check_hmc(h);
+  if (!rc) rc = enqueue_force(h, dtau, nullptr, -1);
+  if (rc) return rc;
+  return ki::sync_stream();
+}
+
+extern "C" int klft_hmc_update_gauge(klft_hmc_t h, double dtau) {
+  int rc = check_hmc(h);
+  if (!rc) rc = enqueue_gauge(h, dtau);
+  if (rc) return rc;
+  return ki::sync_stream();
+}
+
+extern "C" int klft_hmc_kinetic(klft_hmc_t h, double *out) {
+  int rc = check_hmc(h);
+  if (rc) return rc;
+  if (!out) return ki::set_error(KLFT_E_ARG, "null output");
+  rc = enqueue_kinetic(h, 1);
+  if (rc) return rc;
+  return ki::read_results(h->f, 1, 1, out);
+}
+
+extern "C" int klft_gauge_action(klft_field_t f, double beta, double *out) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (!out) return ki::set_error(KLFT_E_ARG, "null output");
+  rc = ki::plaquette_raw(f, 2, nullptr);
+  double raw = 0.0;
+  if (!rc) rc = ki::read_results(f, 2, 1, &raw);
+  if (rc) return rc;
+  *out = -(beta / (double)f->nc) * raw;  // GaugeMonomial.hpp:19-25
+  return KLFT_OK;
+}
+
+extern "C" int klft_hmc_integrate(klft_hmc_t h) {
+  int rc = check_hmc(h);
+  if (!rc) rc = enqueue_leapfrog(h, -1);
+  if (rc) return rc;
+  return ki::sync_stream();
+}
+
+// HMC::hmc_step (HMC.hpp:48-71) with the monomials in the order of
+// lib/HMC_SU2.cpp:46-47 (kinetic, then gauge).
+extern "C" int klft_hmc_trajectory(klft_hmc_t h, uint64_t traj, double *dH, int *accepted) {
+  int rc = check_hmc(h);
+  if (rc) return rc;
+  klft_field_t f = h->f;
+  rc = enqueue_refresh(h, traj, 4);  // K_old
+  if (rc) return rc;
+  KLFT_HIP(hipMemcpyAsync(h->Ubak, f->links, f->link_bytes, hipMemcpyDeviceToDevice, ki::stream()));
+  rc = ki::plaquette_raw(f, 5, nullptr);  // G_old (raw)
+  if (!rc) rc = enqueue_leapfrog(h, 6);   // K_new fused into the last half step
+  if (!rc) rc = ki::plaquette_raw(f, 7, nullptr);
+  double r[4];
+  if (!rc) rc = ki::read_results(f, 4, 4, r);
+  if (rc) return rc;
+  const double c = -(h->p.beta / (double)f->nc);
+  const double K0 = r[0], G0 = c * r[1], K1 = r[2], G1 = c * r[3];
+  double d = 0.0;
+  d += K1 - K0;
+  d += G1 - G0;
+  int acc = 1;
+  if (d > 0.0) {
+    klft_rng::Stream st(h->p.seed, klft_rng::TAG_ACCEPT, 0, (uint32_t)traj, 0, 0, 0);
+    if (st.u01() > std::exp(-d)) acc = 0;
+  }
+  if (!acc) {
+    KLFT_HIP(hipMemcpyAsync(f->links, h->Ubak, f->link_bytes, hipMemcpyDeviceToDevice, ki::stream()));
+    rc = ki::sync_stream();
+    if (rc) return rc;
+  }
+  if (dH) *dH = d;
+  if (accepted) *accepted = acc;
+  return KLFT_OK;
+}

@@ -1,0 +1,147 @@
+// Host-side internals shared by the libklft_hip translation units: handle
+// layout, error reporting, the library stream and the runtime -> template
+// dispatch over (precision, Nc, storage compression, nd).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <string>
+#include <type_traits>
+
+#include "../../include/klft_hip.h"
+#include "device.hpp"
+
+struct klft_field_s {
+  int nd = 0, nc = 0, storage = 0;
+  int dims[4] = {1, 1, 1, 1};
+  kd::Geom g{};
+  void *links = nullptr;  // SoA, sublattice-major (device.hpp)
+  size_t link_bytes = 0;
+  int ncomp = 0;          // complex components stored per link
+  size_t cbytes = 0;      // bytes per complex component
+  // reduction workspace: per-block partials, a ticket and result slots
+  double *partials = nullptr;
+  unsigned int *ticket = nullptr;
+  double *result = nullptr;  // 16 doubles
+  unsigned long long *counter = nullptr;  // 8 counters
+  double *host_result = nullptr;          // pinned, 16 doubles
+};
+
+namespace klft_internal {
+
+constexpr int kMaxPartials = 1 << 16;
+
+int set_error(int code, const char *fmt, ...);
+int ensure_init();
+hipStream_t stream();
+int hip_fail(hipError_t e, const char *what);
+
+#define KLFT_HIP(call)                                          \
+  do {                                                          \
+    hipError_t e_ = (call);                                     \
+    if (e_ != hipSuccess) return klft_internal::hip_fail(e_, #call); \
+  } while (0)
+
+kd::Geom make_geom(int nd, const int *dims);
+// enqueue-only helpers (no host synchronisation)
+int plaquette_raw(klft_field_t f, int slot, double *dev_per_site);  // sum into f->result[slot]
+int read_results(klft_field_t f, int first, int n, double *out);    // synchronises
+int enqueue_reunit(klft_field_t f);
+int enqueue_sweep(klft_field_t f, const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
+                  unsigned long long *counter, int *visit);
+int check_field(const klft_field_s *f);
+int sync_stream();
+
+template <int V>
+using ic = std::integral_constant<int, V>;
+template <typename T>
+struct tag {
+  using type = T;
+};
+
+// fn(tag<R>, ic<NC>, ic<COMP>, ic<ND>)
+template <class F>
+int dispatch(int storage, int nc, int nd, F &&fn) {
+  auto by_nd = [&](auto rt, auto nct, auto ct) -> int {
+    switch (nd) {
+      case 2: return fn(rt, nct, ct, ic<2>{});
+      case 3: return fn(rt, nct, ct, ic<3>{});
+      case 4: return fn(rt, nct, ct, ic<4>{});
+    }
+    return set_error(KLFT_E_ARG, "nd=%d unsupported", nd);
+  };
+  auto by_nc = [&](auto rt, auto ct) -> int {
+    switch (nc) {
+      case 1: return by_nd(rt, ic<1>{}, ic<0>{});
+      case 2: return by_nd(rt, ic<2>{}, ct);
+      case 3: return by_nd(rt, ic<3>{}, ct);
+    }
+    return set_error(KLFT_E_ARG, "Nc=%d unsupported", nc);
+  };
+  switch (storage) {
+    case KLFT_STORAGE_F64: return by_nc(tag<double>{}, ic<0>{});
+    case KLFT_STORAGE_F32: return by_nc(tag<float>{}, ic<0>{});
+    case KLFT_STORAGE_F32_ROWS: return by_nc(tag<float>{}, ic<1>{});
+  }
+  return set_error(KLFT_E_ARG, "storage=%d unsupported", storage);
+}
+
+#define KLFT_UNPACK(rt, nct, ct, ndt)              \
+  using R = typename decltype(rt)::type;           \
+  constexpr int NC = decltype(nct)::value;         \
+  constexpr bool COMP = decltype(ct)::value != 0;  \
+  constexpr int ND = decltype(ndt)::value;         \
+  (void)0
+
+template <typename R, int NC, bool COMP>
+kd::Links<R, NC, COMP> links_of(const klft_field_s *f) {
+  kd::Links<R, NC, COMP> l;
+  l.p = reinterpret_cast<kd::cx<R> *>(f->links);
+  l.V = f->g.V;
+  return l;
+}
+
+inline unsigned int blocks_for(long long n, int block) {
+  return (unsigned int)((n + block - 1) / block);
+}
+
+// workgroups are dispatched round-robin over the 8 XCDs; this gives each XCD
+// a contiguous range of logical block indices so neighbouring tiles share an
+// L2.
+}  // namespace klft_internal
+
+namespace kd {
+__device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int nb) {
+  const unsigned int per = nb / 8u, rem = nb % 8u;
+  const unsigned int x = b % 8u, k = b / 8u;
+  return (x < rem ? x * (per + 1u) : rem * (per + 1u) + (x - rem) * per) + k;
+}
+// Threads are laid out so that one wave covers 64 consecutive sites of one
+// sublattice (coalesced) and four consecutive logical blocks cover all
+// sublattices of a 64-site chunk, with XCD-contiguous logical blocks.
+template <int ND>
+__device__ __forceinline__ bool chunk_site(const Geom &g, unsigned int lb, int *x, long long *s) {
+  constexpr int NSUB = 1 << ND;
+  const long long t = (long long)lb * 256 + threadIdx.x;
+  const long long chunk = t / (64 * NSUB);
+  const int sub = (int)((t / 64) % NSUB);
+  const long long r = chunk * 64 + (t & 63);
+  if (r >= g.Vh) return false;
+  long long q = r;
+#pragma unroll
+  for (int d = ND - 1; d >= 0; --d) {
+    const int y = (int)(q % g.H[d]);
+    q /= g.H[d];
+    x[d] = 2 * y + ((sub >> (ND - 1 - d)) & 1);
+  }
+  *s = (long long)sub * g.Vh + r;
+  return true;
+}
+
+template <int ND>
+__host__ __device__ __forceinline__ long long chunk_threads(const Geom &g) {
+  return ((g.Vh + 63) / 64) * 64 * (1LL << ND);
+}
+
+}  // namespace kd

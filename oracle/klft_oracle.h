@@ -1,0 +1,106 @@
+/*
+ * klft_oracle.h -- FP64 CPU restatement of the klft gauge-update and
+ * measurement path.  TEST INFRASTRUCTURE ONLY: this library is the checker
+ * used by tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of
+ * bench.py.  It is never linked into, loaded by or called from the product
+ * library (klft_amd/csrc).
+ *
+ * Data layout of every gauge field argument: the reference's OpenMP
+ * (LayoutRight) array-of-structs image,
+ *   complex offset = ((site * nd + mu) * nc + c1) * nc + c2,
+ *   site = lexicographic index with i0 slowest: ((i0*L1 + i1)*L2 + i2)*L3 + i3,
+ * stored as interleaved (re, im) doubles.
+ * Momenta / forces: double offset = (site * nd + mu) * ngen + a, with
+ * ngen = 1 (U(1)), 3 (SU(2)), 8 (SU(3)).
+ *
+ * Random numbers: Philox4x32-10 keyed by the 64-bit seed; see
+ * orc_philox4x32_10 and the stream layout documented in klft_oracle.c.
+ */
+#ifndef KLFT_ORACLE_H
+#define KLFT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- random numbers ----------------------------------------------------- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* n doubles / floats from the stream (tag, c0, c1, c2, chain, mu) */
+void orc_stream_doubles(uint64_t seed, uint32_t tag, uint32_t c0, uint32_t c1,
+                        uint32_t c2, uint32_t chain, uint32_t mu, int n, double *out);
+
+/* --- SU(N) algebra on single matrices (interleaved re/im, row major) ---- */
+void orc_matmul(int nc, const double *a, const double *b, double *c);
+void orc_dagger(int nc, const double *a, double *c);
+void orc_restore_sun(int nc, double *a);                 /* SUN.hpp:256-340 */
+/* randSUN from the Metropolis stream of (seed, sweep, site, mu, hit):
+ * writes the proposal and returns the number of 32-bit words consumed */
+int orc_rand_sun(int nc, double delta, uint64_t seed, uint32_t sweep,
+                 uint32_t site, uint32_t mu, uint32_t hit, double *r);
+/* exp(i Q), Q = sum_a v_a T_a (T = 1, sigma, Gell-Mann) */
+void orc_exp_iq(int nc, const double *v, double *out);
+
+/* --- field initialisation --------------------------------------------- */
+void orc_field_identity(int nd, int nc, const int *dims, double *U);
+/* U = exp(i eps Q), Q = sum_a h_a T_a / 2, h_a ~ N(0,1) keyed (seed, site, mu) */
+void orc_field_random_algebra(int nd, int nc, const int *dims, double eps,
+                              uint64_t seed, double *U);
+
+/* --- measurement --------------------------------------------------------- */
+void orc_staple(int nd, int nc, const int *dims, const double *U, long site,
+                int mu, double *S);
+void orc_staple_field(int nd, int nc, const int *dims, const double *U, double *S);
+double orc_plaquette(int nd, int nc, const int *dims, const double *U,
+                     int normalize, double *per_site /* V doubles or NULL */);
+void orc_wilson_loop_mu_nu(int nd, int nc, const int *dims, const double *U,
+                           int mu, int nu, const int *pairs, int npairs,
+                           int normalize, double *out, double *per_site_last);
+int orc_wilson_loop_temporal(int nd, int nc, const int *dims, const double *U,
+                             const int *pairs, int npairs, int normalize,
+                             double *out);
+
+/* --- Metropolis ------------------------------------------------------------ */
+/* one sweep (2^nd sublattice passes); returns the acceptance rate.
+ * visit: optional int32 array of V entries receiving the pass index that
+ * updated each site (schedule dump). */
+double orc_metropolis_sweep(int nd, int nc, const int *dims, double *U,
+                            int nhits, double beta, double delta,
+                            uint64_t seed, uint32_t sweep, long *naccepted,
+                            int *visit);
+
+/* --- HMC ------------------------------------------------------------------- */
+int orc_ngen(int nc);
+void orc_refresh_momenta(int nd, int nc, const int *dims, uint64_t seed,
+                         uint32_t traj, double *P);
+void orc_force(int nd, int nc, const int *dims, const double *U, double beta,
+               double *F);
+void orc_update_momentum(int nd, int nc, const int *dims, double *P,
+                         const double *F, double dtau);
+void orc_update_gauge(int nd, int nc, const int *dims, double *U,
+                      const double *P, double dtau);
+void orc_restore_field(int nd, int nc, const int *dims, double *U);
+double orc_kinetic(int nd, int nc, const int *dims, const double *P);
+double orc_gauge_action(int nd, int nc, const int *dims, const double *U,
+                        double beta);
+void orc_leapfrog(int nd, int nc, const int *dims, double *U, double *P,
+                  double beta, double tau, int nsteps);
+/* one HMC trajectory; returns 1 if accepted.  dH, plaq may be NULL */
+int orc_hmc_trajectory(int nd, int nc, const int *dims, double *U, double beta,
+                       double tau, int nsteps, uint64_t seed, uint32_t traj,
+                       double *dH, double *plaq);
+
+/* --- PTBC defect (links scaled by c on the defect) ------------------------- */
+void orc_defect_field(int nd, const int *dims, int defect_length,
+                      const int *defect_pos, double cr, double *C);
+double orc_plaquette_defect(int nd, int nc, const int *dims, const double *U,
+                            const double *C, int normalize);
+double orc_metropolis_sweep_defect(int nd, int nc, const int *dims, double *U,
+                                   const double *C, int nhits, double beta,
+                                   double delta, uint64_t seed, uint32_t sweep,
+                                   uint32_t chain, long *naccepted);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

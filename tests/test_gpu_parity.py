@@ -1,0 +1,184 @@
+"""GPU parity: the HIP library (through the C ABI) against the FP64 oracle.
+
+Tolerances: FP64 per-site values 1e-12 (entries are O(1)); global sums are
+compared relative to the sum of absolute per-site terms (the reduction order
+differs between the GPU tree and the oracle's pairwise sum).  Index and
+schedule data must be bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+klft = pytest.importorskip("klft_amd")
+
+SHAPES = [(2, (6, 4)), (3, (4, 6, 4)), (4, (4, 4, 6, 4))]
+
+
+def seeded(nd, nc, dims, eps, seed):
+    return orc.random_algebra(nd, nc, dims, eps, seed)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", SHAPES)
+def test_upload_download_roundtrip(nd, nc, dims):
+    U = seeded(nd, nc, dims, 0.7, 11)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    assert np.array_equal(g.download(), U)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", SHAPES)
+def test_random_algebra_init_matches_oracle(nd, nc, dims):
+    U = seeded(nd, nc, dims, 0.5, 7)
+    g = klft.GaugeField(nd, nc, dims, "f64").set_random_algebra(0.5, 7)
+    np.testing.assert_allclose(g.download(), U, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("nd,dims", SHAPES + [(4, (8, 6, 4, 10))])
+def test_schedule_bit_exact(nd, dims):
+    U = orc.identity(nd, 1, dims)
+    _, _, visit = orc.metropolis_sweep(U, nd, 1, dims, 0, 1.0, 0.1, 1, 0, visit=True)
+    g = klft.GaugeField(nd, 1, dims, "f64").set_identity()
+    v = klft.metropolis_schedule(g)
+    assert np.array_equal(v, visit)
+    assert set(np.unique(v)) == set(range(2 ** nd))
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", SHAPES)
+def test_plaquette_cold_is_one(nd, nc, dims):
+    g = klft.GaugeField(nd, nc, dims, "f64").set_identity()
+    assert klft.GaugePlaquette(g) == 1.0
+
+
+@pytest.mark.parametrize("eps", [0.1, 0.5, 1.0])
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", SHAPES)
+def test_plaquette_parity(nd, nc, dims, eps):
+    U = seeded(nd, nc, dims, eps, 100)
+    ref, ref_ps = orc.plaquette(U, nd, nc, dims, per_site=True)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    got, ps = klft.GaugePlaquette(g, per_site=True)
+    np.testing.assert_allclose(ps, ref_ps, rtol=0, atol=1e-12)
+    norm = np.prod(dims) * (nd * (nd - 1) // 2) * nc
+    assert abs(got - ref) <= 1e-12 * np.abs(ref_ps).sum() / norm + 1e-15
+    raw = klft.GaugePlaquette(g, normalize=False)
+    assert abs(raw - orc.plaquette(U, nd, nc, dims, normalize=False)) <= 1e-12 * np.abs(ref_ps).sum()
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", SHAPES)
+def test_staple_field_parity(nd, nc, dims):
+    U = seeded(nd, nc, dims, 0.8, 5)
+    S = orc.staple_field(U, nd, nc, dims)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    out = klft.stapleField(g).download()
+    np.testing.assert_allclose(out, S, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", [(3, (4, 6, 8)), (4, (4, 4, 6, 8))])
+def test_wilson_mu_nu_parity(nd, nc, dims):
+    U = seeded(nd, nc, dims, 0.6, 21)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    pairs = [(1, 1), (2, 2), (3, 4), (4, 3), (2, 5), (0, 2)]
+    for mu, nu in [(0, 1), (1, 2), (nd - 1, 0)]:
+        ref, ref_ps = orc.wilson_mu_nu(U, nd, nc, dims, mu, nu, pairs, per_site=True)
+        got, ps = klft.WilsonLoop_mu_nu(g, mu, nu, pairs, per_site=True)
+        np.testing.assert_allclose(ps, ref_ps, rtol=0, atol=1e-12)
+        scale = np.abs(ref_ps).sum() / np.prod(dims) / nc
+        for (m, n, Lm, Ln, w), r in zip(got, ref):
+            assert (m, n) == (mu, nu)
+            assert abs(w - r) <= 1e-12 * max(scale, 1.0)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_wilson_temporal_parity(nc):
+    nd, dims = 4, (4, 6, 4, 8)
+    U = seeded(nd, nc, dims, 0.4, 3)
+    pairs = [(2, 2), (3, 4), (4, 3), (4, 4)]
+    ref = orc.wilson_temporal(U, nd, nc, dims, pairs)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    got = klft.WilsonLoop_temporal(g, pairs)
+    for (L, T, w), r, p in zip(got, ref, pairs):
+        assert (L, T) == p
+        assert abs(w - r) <= 1e-12
+
+
+def test_wilson_cold_is_one():
+    g = klft.GaugeField(4, 3, (4, 4, 4, 8), "f64").set_identity()
+    for L, T, w in klft.WilsonLoop_temporal(g, [(1, 1), (2, 3), (4, 8)]):
+        assert abs(w - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("nc,beta", [(1, 2.0), (2, 2.3), (3, 5.5)])
+@pytest.mark.parametrize("nd,dims", SHAPES)
+def test_metropolis_fp64_tracks_oracle(nd, nc, dims, beta):
+    """Same keyed stream: the GPU sweep follows the oracle's decisions."""
+    U = seeded(nd, nc, dims, 0.3, 9)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    p = klft.MetropolisParams(nHits=4, beta=beta, delta=0.3, seed=4242)
+    Uo = U.copy()
+    for sweep in range(3):
+        acc_o, n_o = orc.metropolis_sweep(Uo, nd, nc, dims, 4, beta, 0.3, 4242, sweep)
+        acc_g, n_g = klft.sweep_Metropolis(g, p, sweep, with_count=True)
+        assert n_g == n_o, (sweep, n_g, n_o)
+        assert acc_g == acc_o
+    np.testing.assert_allclose(g.download(), Uo, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_hmc_momenta_and_force_parity(nc):
+    nd, dims = 4, (4, 4, 4, 6)
+    U = seeded(nd, nc, dims, 0.7, 31)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    h = klft.HMC(g, beta=2.0, tau=1.0, nsteps=5, seed=77)
+    h.refresh_momenta(3)
+    np.testing.assert_allclose(h.momenta(), orc.refresh_momenta(nd, nc, dims, 77, 3), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(h.force(), orc.force(U, nd, nc, dims, 2.0), rtol=0, atol=1e-12)
+    P = orc.refresh_momenta(nd, nc, dims, 77, 3)
+    assert abs(h.kinetic() - orc.kinetic(P, nd, nc, dims)) < 1e-10
+    assert abs(klft.gauge_action(g, 2.0) - orc.gauge_action(U, nd, nc, dims, 2.0)) < 1e-9
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_hmc_update_gauge_parity(nc):
+    nd, dims = 4, (4, 4, 4, 4)
+    U = seeded(nd, nc, dims, 0.7, 32)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    h = klft.HMC(g, beta=2.0, tau=1.0, nsteps=5, seed=78)
+    P = orc.refresh_momenta(nd, nc, dims, 78, 0)
+    h.set_momenta(P)
+    h.update_gauge(0.1)
+    Uo = U.copy()
+    orc.update_gauge(Uo, P, nd, nc, dims, 0.1)
+    np.testing.assert_allclose(g.download(), Uo, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("nc,beta", [(1, 1.0), (2, 2.3), (3, 5.0)])
+def test_hmc_trajectory_parity(nc, beta):
+    nd, dims = 4, (4, 4, 4, 4)
+    U = seeded(nd, nc, dims, 0.5, 40)
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    h = klft.HMC(g, beta=beta, tau=1.0, nsteps=10, seed=99)
+    Uo = U.copy()
+    for traj in range(2):
+        acc_o, dH_o, _ = orc.hmc_trajectory(Uo, nd, nc, dims, beta, 1.0, 10, 99, traj)
+        dH, acc = h.trajectory(traj)
+        assert abs(dH - dH_o) < 1e-8 * max(1.0, abs(dH_o))
+        assert acc == acc_o
+    np.testing.assert_allclose(g.download(), Uo, rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32_rows"])
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_fp32_plaquette_within_1e6(nc, storage):
+    nd, dims = 4, (4, 4, 6, 8)
+    U = seeded(nd, nc, dims, 0.5, 13)
+    ref = orc.plaquette(U, nd, nc, dims)
+    g = klft.GaugeField(nd, nc, dims, storage).set_random_algebra(0.5, 13).reunitarize()
+    got = klft.GaugePlaquette(g)
+    assert abs(got - ref) < 1e-6
