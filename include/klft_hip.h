@@ -191,6 +191,17 @@ int klft_ptbc_labels(klft_batch_t b, int *labels);
  * index `coupling` (sum over all plaquettes, defect-scaled links) */
 int klft_ptbc_action(klft_batch_t b, int replica, int coupling, double *out);
 
+/* ---- autotuner (tune_and_launch_for + read/writeTuneCache, Tuner.hpp:73-358) ---------- */
+/* Enable / disable variant tuning (default: the KLFT_TUNING environment
+ * variable).  Tuning times every launch variant of a kernel on the first call
+ * for a field shape, without changing the live field. */
+int klft_tuning(int on);
+/* Cache file lines: "<rank> <key> <variant> <ns> 0 .." (the reference's
+ * "<rank> <key> t0 .. t_{rank-1}" shape).  A missing file is reported, not fatal. */
+int klft_tune_cache_read(const char *path);
+int klft_tune_cache_write(const char *path);
+int klft_tune_lookup(const char *key, int *variant);
+
 /* ---- drop-in driver (klft::Metropolis, lib/Metropolis.cpp:36-166) ------------------ */
 /* Reads the YAML input, runs the sweeps and writes the observable files
  * exactly as the reference's metropolis binary.  Returns 0 or -1. */

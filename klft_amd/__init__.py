@@ -107,6 +107,10 @@ _SIGS = {
     "klft_ptbc_labels": (C.c_int, [_p, _ip]),
     "klft_ptbc_action": (C.c_int, [_p, C.c_int, C.c_int, _dp]),
     "klft_metropolis_main": (C.c_int, [C.c_char_p]),
+    "klft_tuning": (C.c_int, [C.c_int]),
+    "klft_tune_cache_read": (C.c_int, [C.c_char_p]),
+    "klft_tune_cache_write": (C.c_int, [C.c_char_p]),
+    "klft_tune_lookup": (C.c_int, [C.c_char_p, _ip]),
 }
 
 EXPORTED = []

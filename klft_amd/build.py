@@ -31,7 +31,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INCLUDE}",
           "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
 
-HIP_SOURCES = ["core.hip", "metropolis.hip", "hmc.hip", "batch.hip"]
+HIP_SOURCES = ["core.hip", "metropolis.hip", "hmc.hip", "tuner.hip", "batch.hip"]
 HOST_SOURCES = ["host/yaml_lite.cpp", "host/observables.cpp", "host/driver.cpp",
                 "host/tuner.cpp"]
 TOOLS = {"metropolis": "tools/metropolis_main.cpp", "hmc": "tools/hmc_main.cpp",

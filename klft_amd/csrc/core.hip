@@ -470,28 +470,21 @@ template <typename R, int NC, bool COMP, int ND>
 __device__ __forceinline__ double plaq_at(const Links<R, NC, COMP> &f, const Geom &g, const int *x,
                                           long long s) {
   double p = 0.0;
-#pragma unroll
+#pragma unroll 1
   for (int mu = 0; mu < ND - 1; ++mu) {
-    int xm[ND];
-    copy_x<ND>(x, xm);
-    xm[mu] = up(x[mu], g.L[mu]);
-    const long long sm = dev_index<ND>(g, xm);
-    const Mat<R, NC> umu = f.load(s, mu);
-#pragma unroll
+    const long long sm = idx_shift<ND>(g, x, mu, 1, -1, 0);
+#pragma unroll 1
     for (int nu = mu + 1; nu < ND; ++nu) {
-      int xn[ND];
-      copy_x<ND>(x, xn);
-      xn[nu] = up(x[nu], g.L[nu]);
-      const Mat<R, NC> lmu = mul(umu, f.load(sm, nu));
-      const Mat<R, NC> lnu = mul(f.load(s, nu), f.load(dev_index<ND>(g, xn), mu));
+      const Mat<R, NC> lmu = mul(f.load(s, mu), f.load(sm, nu));
+      const Mat<R, NC> lnu = mul(f.load(s, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
       p += (double)retr_mul_ad(lmu, lnu);
     }
   }
   return p;
 }
 
-template <typename R, int NC, bool COMP, int ND>
-__global__ __launch_bounds__(256) void k_plaquette(Links<R, NC, COMP> f, Geom g, double *partials,
+template <typename R, int NC, bool COMP, int ND, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_plaquette(Links<R, NC, COMP> f, Geom g, double *partials,
                                                     unsigned int *ticket, double *out,
                                                     double *per_site) {
   __shared__ double lds[8];
@@ -509,17 +502,25 @@ __global__ __launch_bounds__(256) void k_plaquette(Links<R, NC, COMP> f, Geom g,
 
 namespace klft_internal {
 // raw (unnormalised) plaquette sum into f->result[slot]; optional per-site
-int plaquette_raw(klft_field_t f, int slot, double *dev_per_site) {
+static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int variant) {
   return dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
     const unsigned nb = blocks_for(chunk_threads<ND>(f->g), 256);
     if (nb > (unsigned)kMaxPartials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
-    hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND>), dim3(nb), dim3(256), 0, stream(),
-                       links_of<R, NC, COMP>(f), f->g, f->partials, f->ticket, f->result + slot,
-                       dev_per_site);
-    KLFT_HIP(hipGetLastError());
-    return KLFT_OK;
+    return with_occ(variant, [&](auto occ) -> int {
+      constexpr int OCC = decltype(occ)::value;
+      hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, OCC>), dim3(nb), dim3(256), 0, stream(),
+                         links_of<R, NC, COMP>(f), f->g, f->partials, f->ticket, f->result + slot,
+                         dev_per_site);
+      KLFT_HIP(hipGetLastError());
+      return KLFT_OK;
+    });
   });
+}
+int plaquette_raw(klft_field_t f, int slot, double *dev_per_site) {
+  const int v = pick_variant("plaquette", f, kOccVariants, 0,
+                             [&](int var) { return launch_plaquette(f, slot, nullptr, var); });
+  return launch_plaquette(f, slot, dev_per_site, v);
 }
 int read_results(klft_field_t f, int first, int n, double *out) {
   KLFT_HIP(hipMemcpyAsync(f->host_result, f->result + first, n * sizeof(double), hipMemcpyDeviceToHost,
@@ -558,15 +559,14 @@ extern "C" int klft_plaquette(klft_field_t f, int normalize, double *out, double
 // ============================================================================
 // staple field (Gauge_Util.hpp:27-97)
 // ============================================================================
-template <typename R, int NC, bool COMP, int ND>
-__global__ __launch_bounds__(256) void k_staple_field(Links<R, NC, COMP> f, Links<R, NC, false> o,
-                                                       Geom g) {
+template <typename R, int NC, bool COMP, int ND, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_staple_field(Links<R, NC, COMP> f,
+                                                                            Links<R, NC, false> o, Geom g) {
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
-  int x[ND];
+  int x[ND], mu;
   long long s;
-  if (!chunk_site<ND>(g, lb, x, &s)) return;
-#pragma unroll 1
-  for (int mu = 0; mu < ND; ++mu) o.store(s, mu, staple<R, NC, COMP, ND>(f, g, x, mu));
+  if (!link_site<ND>(g, lb, x, &s, &mu)) return;
+  o.store(s, mu, staple_lowreg<R, NC, COMP, ND>(f, g, x, mu));
 }
 
 extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
@@ -578,14 +578,20 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
     return ki::set_error(KLFT_E_ARG, "staple output must be a full-storage field of the same precision and shape");
   for (int d = 0; d < f->nd; ++d)
     if (out->dims[d] != f->dims[d]) return ki::set_error(KLFT_E_ARG, "staple output extents differ");
-  rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
-    KLFT_UNPACK(rt, nct, ct, ndt);
-    const unsigned nb = ki::blocks_for(chunk_threads<ND>(f->g), 256);
-    hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND>), dim3(nb), dim3(256), 0, ki::stream(),
-                       ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
-    KLFT_HIP(hipGetLastError());
-    return KLFT_OK;
-  });
+  auto launch = [&](int variant) {
+    return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+      KLFT_UNPACK(rt, nct, ct, ndt);
+      const unsigned nb = (unsigned)link_blocks<ND>(f->g);
+      return ki::with_occ(variant, [&](auto occ) -> int {
+        constexpr int OCC = decltype(occ)::value;
+        hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND, OCC>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
+                           ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
+        KLFT_HIP(hipGetLastError());
+        return KLFT_OK;
+      });
+    });
+  };
+  rc = launch(ki::pick_variant("staple_field", f, ki::kOccVariants, 0, launch));
   if (rc) return rc;
   return ki::sync_stream();
 }
@@ -632,17 +638,13 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
     }
     x[nu] = k;
   }
-  const int Lm = g.L[mu];
   Mat<R, NC> F = identity<R, NC>();
   int Lcur = 0;
   for (int li = 0; li < nL; ++li) {
     const int L = Ls[li];
     if (valid) {
       while (Lcur < L) {
-        int xs[ND];
-        copy_x<ND>(x, xs);
-        xs[mu] = shiftk(x[mu], Lcur, Lm);
-        F = mul(F, f.load(dev_index<ND>(g, xs), mu));
+        F = mul(F, f.load(idx_shift_k<ND>(g, x, mu, Lcur, -1, 0), mu));
         ++Lcur;
       }
     }
@@ -650,19 +652,13 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
     lds[tid] = F;
     __syncthreads();
     Mat<R, NC> D = F;
-    int xa[ND], xb[ND];
-    copy_x<ND>(x, xa);
-    copy_x<ND>(x, xb);
-    xb[mu] = shiftk(x[mu], L, Lm);
     for (int T = 0; T <= Tmax; ++T) {
       double w = 0.0;
       if (valid) {
         if (T > 0) {
-          const Mat<R, NC> a = f.load(dev_index<ND>(g, xa), nu);
-          const Mat<R, NC> b = f.load(dev_index<ND>(g, xb), nu);
+          const Mat<R, NC> a = f.load(idx_shift_k<ND>(g, x, nu, T - 1, -1, 0), nu);
+          const Mat<R, NC> b = f.load(idx_shift_k<ND>(g, x, mu, L, nu, T - 1), nu);
           D = mul(mul_da(a, D), b);
-          xa[nu] = up(xa[nu], Ln);
-          xb[nu] = up(xb[nu], Ln);
         }
         int kt = k + T;
         kt %= Ln;

@@ -371,6 +371,44 @@ __device__ __forceinline__ int shiftk(int v, int k, int L) {
   return r < 0 ? r + L : r;
 }
 
+// device index of x + s1 e_m1 + s2 e_m2 for s1, s2 in {-1, 0, +1} (m2 = -1:
+// none).  The coordinate loop is unrolled and the runtime directions are
+// compared, not used as array indices, so nothing spills to scratch.
+template <int ND>
+__device__ __forceinline__ long long idx_shift(const Geom& g, const int* x, int m1, int s1, int m2,
+                                               int s2) {
+  int sub = 0;
+  long long r = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    int v = x[d];
+    const int Ld = g.L[d];
+    if (d == m1) v += s1;
+    if (d == m2) v += s2;
+    v = v < 0 ? v + Ld : (v >= Ld ? v - Ld : v);
+    sub |= (v & 1) << (ND - 1 - d);
+    r += (long long)(v >> 1) * g.hs[d];
+  }
+  return (long long)sub * g.Vh + r;
+}
+// the same for arbitrary non-negative shifts k1, k2
+template <int ND>
+__device__ __forceinline__ long long idx_shift_k(const Geom& g, const int* x, int m1, int k1, int m2,
+                                                 int k2) {
+  int sub = 0;
+  long long r = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    int v = x[d];
+    if (d == m1) v += k1;
+    if (d == m2) v += k2;
+    v %= g.L[d];
+    sub |= (v & 1) << (ND - 1 - d);
+    r += (long long)(v >> 1) * g.hs[d];
+  }
+  return (long long)sub * g.Vh + r;
+}
+
 // ---------------------------------------------------------------------------
 // link field access
 // ---------------------------------------------------------------------------
@@ -409,38 +447,58 @@ template <typename R, int NC, bool COMP, int ND>
 __device__ __forceinline__ Mat<R, NC> staple(const Links<R, NC, COMP>& f, const Geom& g,
                                              const int* x, int mu) {
   Mat<R, NC> t = zero<R, NC>();
-  int xp[ND];
-  copy_x<ND>(x, xp);
-  xp[mu] = up(x[mu], g.L[mu]);
   const long long s_x = dev_index<ND>(g, x);
-  const long long s_xpmu = dev_index<ND>(g, xp);
+  const long long s_xpmu = idx_shift<ND>(g, x, mu, 1, -1, 0);
 #pragma unroll
   for (int nu = 0; nu < ND; ++nu) {
     if (nu == mu) continue;
-    int xn[ND];
-    copy_x<ND>(x, xn);
-    xn[nu] = up(x[nu], g.L[nu]);
     const Mat<R, NC> a = f.load(s_xpmu, nu);
-    const Mat<R, NC> b = f.load(dev_index<ND>(g, xn), mu);
+    const Mat<R, NC> b = f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu);
     const Mat<R, NC> c = f.load(s_x, nu);
     add_to(t, mul_ad(mul_ad(a, b), c));
   }
 #pragma unroll
   for (int nu = 0; nu < ND; ++nu) {
     if (nu == mu) continue;
-    int xa[ND], xb[ND];
-    copy_x<ND>(xp, xa);
-    xa[nu] = dn(xp[nu], g.L[nu]);
-    copy_x<ND>(x, xb);
-    xb[nu] = dn(x[nu], g.L[nu]);
-    const long long s_xmnu = dev_index<ND>(g, xb);
-    const Mat<R, NC> a = f.load(dev_index<ND>(g, xa), nu);
+    const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+    const Mat<R, NC> a = f.load(idx_shift<ND>(g, x, mu, 1, nu, -1), nu);
     const Mat<R, NC> b = f.load(s_xmnu, mu);
     const Mat<R, NC> c = f.load(s_xmnu, nu);
     add_to(t, mul(mul_dd(a, b), c));
   }
   return t;
 }
+
+// The same staple with one term in flight at a time: the compiler cannot
+// hoist all 2(nd-1)*3 link loads, which keeps register use low enough for
+// several waves per SIMD (used by the one-thread-per-link kernels).
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ Mat<R, NC> staple_lowreg(const Links<R, NC, COMP>& f, const Geom& g,
+                                                    const int* x, int mu) {
+  Mat<R, NC> t = zero<R, NC>();
+  const long long s_x = dev_index<ND>(g, x);
+  const long long s_xpmu = idx_shift<ND>(g, x, mu, 1, -1, 0);
+#pragma unroll 1
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == mu) continue;
+    const Mat<R, NC> ab = mul_ad(f.load(s_xpmu, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
+    add_to(t, mul_ad(ab, f.load(s_x, nu)));
+  }
+#pragma unroll 1
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == mu) continue;
+    const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+    const Mat<R, NC> ab = mul_dd(f.load(idx_shift<ND>(g, x, mu, 1, nu, -1), nu), f.load(s_xmnu, mu));
+    add_to(t, mul(ab, f.load(s_xmnu, nu)));
+  }
+  return t;
+}
+
+// waves per SIMD requested from the register allocator
+template <typename R, int NC>
+struct Occ {
+  static constexpr int value = (sizeof(R) == 8 && NC == 3) ? 3 : 4;
+};
 
 // ---------------------------------------------------------------------------
 // Metropolis proposals, randSUN<1,2,3> (SUN.hpp:163-251), drawn from the
@@ -696,7 +754,7 @@ __device__ __forceinline__ void grid_finish(double block_total, double* partials
   __syncthreads();
   if (!last) return;
   double v = 0.0;
-  for (unsigned int i = threadIdx.x; i < gridDim.x; i += BLOCK)
+  for (unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x)
     v += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   v = block_sum<BLOCK>(v, lds);
   if (threadIdx.x == 0) {

@@ -69,36 +69,36 @@ __global__ __launch_bounds__(256) void k_refresh(R *P, Geom g, uint64_t seed, ui
 // P -= dtau * F with F recomputed from the current links.  Fout (optional):
 // write F in host AoS order instead of updating P.  kin (optional): also
 // reduce 1/2 sum p^2 of the updated momenta.
-template <typename R, int NC, bool COMP, int ND>
-__global__ __launch_bounds__(256) void k_force(Links<R, NC, COMP> f, R *P, Geom g, R coef, R dtau,
-                                                double *Fout, double *partials,
-                                                unsigned int *ticket, double *kin) {
+template <typename R, int NC, bool COMP, int ND, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, Geom g,
+                                                                     R coef, R dtau, double *Fout,
+                                                                     double *partials,
+                                                                     unsigned int *ticket, double *kin) {
   __shared__ double lds[8];
   constexpr int NG = Ngen<NC>::value;
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
-  int x[ND];
+  int x[ND], mu;
   long long s;
   double k = 0.0;
-  if (chunk_site<ND>(g, lb, x, &s)) {
-#pragma unroll 1
-    for (int mu = 0; mu < ND; ++mu) {
-      const Mat<R, NC> S = staple<R, NC, COMP, ND>(f, g, x, mu);
-      const Mat<R, NC> M = mul(f.load(s, mu), S);
-      R F[NG];
-      project(M, coef, F);
-      if (Fout) {
-        const long long site = lex_index<ND>(g, x);
+  if (link_site<ND>(g, lb, x, &s, &mu)) {
+    const Mat<R, NC> S = staple_lowreg<R, NC, COMP, ND>(f, g, x, mu);
+    const Mat<R, NC> M = mul(f.load(s, mu), S);
+    R F[NG];
+    project(M, coef, F);
+    if (Fout) {
+      const long long site = lex_index<ND>(g, x);
 #pragma unroll
-        for (int a = 0; a < NG; ++a) Fout[(site * ND + mu) * NG + a] = (double)F[a];
-      } else {
+      for (int a = 0; a < NG; ++a) Fout[(site * ND + mu) * NG + a] = (double)F[a];
+    } else {
+      double n2 = 0.0;
 #pragma unroll
-        for (int a = 0; a < NG; ++a) {
-          R *q = P + ((long long)mu * NG + a) * g.V + s;
-          const R v = *q - dtau * F[a];
-          *q = v;
-          k += 0.5 * ((double)v * (double)v);
-        }
+      for (int a = 0; a < NG; ++a) {
+        R *q = P + ((long long)mu * NG + a) * g.V + s;
+        const R v = *q - dtau * F[a];
+        *q = v;
+        n2 += (double)v * (double)v;
       }
+      k = 0.5 * n2;
     }
   }
   if (kin) {
@@ -190,19 +190,29 @@ int enqueue_refresh(klft_hmc_t h, uint64_t traj, int slot) {
   });
 }
 
-int enqueue_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot) {
+int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int variant) {
   klft_field_t f = h->f;
   return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
-    const unsigned nb = ki::blocks_for(chunk_threads<ND>(f->g), 256);
+    const unsigned nb = (unsigned)link_blocks<ND>(f->g);
     if (nb > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
-    hipLaunchKernelGGL((k_force<R, NC, COMP, ND>), dim3(nb), dim3(256), 0, ki::stream(),
-                       ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC),
-                       (R)dtau, Fout, f->partials, f->ticket,
-                       kin_slot >= 0 ? f->result + kin_slot : (double *)nullptr);
-    KLFT_HIP(hipGetLastError());
-    return KLFT_OK;
+    return ki::with_occ(variant, [&](auto occ) -> int {
+      constexpr int OCC = decltype(occ)::value;
+      hipLaunchKernelGGL((k_force<R, NC, COMP, ND, OCC>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
+                         ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC),
+                         (R)dtau, Fout, f->partials, f->ticket,
+                         kin_slot >= 0 ? f->result + kin_slot : (double *)nullptr);
+      KLFT_HIP(hipGetLastError());
+      return KLFT_OK;
+    });
   });
+}
+
+// the tuning launcher uses dtau = 0, which leaves the momenta unchanged
+int enqueue_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot) {
+  const int v = ki::pick_variant("force", h->f, ki::kOccVariants, 0,
+                                 [&](int var) { return launch_force(h, 0.0, nullptr, -1, var); });
+  return launch_force(h, dtau, Fout, kin_slot, v);
 }
 
 int enqueue_gauge(klft_hmc_t h, double dtau) {

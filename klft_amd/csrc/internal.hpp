@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstddef>
+#include <functional>
 #include <string>
 #include <type_traits>
 
@@ -55,6 +56,23 @@ int sync_stream();
 
 template <int V>
 using ic = std::integral_constant<int, V>;
+
+// ---- autotuner (tuner.hip) ----
+bool tuning_enabled();
+std::string tune_key(const char *kernel, const klft_field_s *f);
+int pick_variant(const char *kernel, const klft_field_s *f, int nvar, int dflt,
+                 const std::function<int(int)> &run);
+
+// occupancy variants: v = 0, 1, 2 -> at least 1, 2, 4 waves per SIMD
+constexpr int kOccVariants = 3;
+template <class F>
+int with_occ(int v, F &&fn) {
+  switch (v) {
+    case 1: return fn(ic<2>{});
+    case 2: return fn(ic<4>{});
+    default: return fn(ic<1>{});
+  }
+}
 template <typename T>
 struct tag {
   using type = T;
@@ -142,6 +160,32 @@ __device__ __forceinline__ bool chunk_site(const Geom &g, unsigned int lb, int *
 template <int ND>
 __host__ __device__ __forceinline__ long long chunk_threads(const Geom &g) {
   return ((g.Vh + 63) / 64) * 64 * (1LL << ND);
+}
+
+// One thread per link: a block of 64*ND threads holds one 64-site chunk of
+// one sublattice, wave w taking direction mu = w, so the ND waves of a block
+// share the chunk's links through L1/L2 while each wave reads coalesced.
+template <int ND>
+__device__ __forceinline__ bool link_site(const Geom &g, unsigned int lb, int *x, long long *s,
+                                          int *mu) {
+  constexpr int NSUB = 1 << ND;
+  const int sub = (int)(lb % NSUB);
+  const long long r = (long long)(lb / NSUB) * 64 + (threadIdx.x & 63);
+  *mu = (int)(threadIdx.x >> 6);
+  if (r >= g.Vh) return false;
+  long long q = r;
+#pragma unroll
+  for (int d = ND - 1; d >= 0; --d) {
+    const int y = (int)(q % g.H[d]);
+    q /= g.H[d];
+    x[d] = 2 * y + ((sub >> (ND - 1 - d)) & 1);
+  }
+  *s = (long long)sub * g.Vh + r;
+  return true;
+}
+template <int ND>
+__host__ __device__ __forceinline__ long long link_blocks(const Geom &g) {
+  return ((g.Vh + 63) / 64) * (1LL << ND);
 }
 
 }  // namespace kd
