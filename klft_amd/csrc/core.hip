@@ -79,15 +79,26 @@ Geom make_geom(int nd, const int *dims) {
     g.V *= g.L[d];
     g.Vh *= g.H[d];
   }
+  // tile extents: 4 where the half extent allows it, else 2, else 1
+  g.TV = 1;
+  g.tvb = 0;
   for (int d = 0; d < 4; ++d) {
-    g.hs[d] = 0;
+    g.tb[d] = 0;
+    if (d < nd) g.tb[d] = (g.H[d] % 4 == 0) ? 2 : ((g.H[d] % 2 == 0) ? 1 : 0);
+    g.NT[d] = g.H[d] >> g.tb[d];
+    g.TV <<= g.tb[d];
+    g.tvb += g.tb[d];
+    g.ts[d] = 0;
     g.ls[d] = 0;
+    g.lsh[d] = 0;
   }
-  g.hs[nd - 1] = 1;
   g.ls[nd - 1] = 1;
+  g.ts[nd - 1] = g.TV;
+  g.lsh[nd - 1] = 0;
   for (int d = nd - 2; d >= 0; --d) {
-    g.hs[d] = g.hs[d + 1] * g.H[d + 1];
     g.ls[d] = g.ls[d + 1] * g.L[d + 1];
+    g.ts[d] = g.ts[d + 1] * g.NT[d + 1];
+    g.lsh[d] = g.lsh[d + 1] + g.tb[d + 1];
   }
   return g;
 }

@@ -310,25 +310,63 @@ __device__ __forceinline__ void reunitarize(Mat<float, N>& a) {
 // ---------------------------------------------------------------------------
 // lattice geometry
 // ---------------------------------------------------------------------------
+// Within each sublattice the half-extent coordinates y_d = x_d >> 1 are
+// ordered in 4D tiles of power-of-two extent (4 where H_d = L_d/2 allows it):
+//   r = sum_d (y_d >> tb_d) ts_d + sum_d (y_d & (T_d - 1)) << lsh_d,
+// tiles in lexicographic order (last dimension fastest), sites inside a tile
+// likewise.  A tile's +-1 neighbours then lie in the same tile of another
+// sublattice, which the launch order keeps resident in L2.
 struct Geom {
   int nd;
   int L[4];
   int H[4];          // L / 2
+  int tb[4];         // log2 of the tile extent
+  int lsh[4];        // log2 of the in-tile stride
+  int NT[4];         // tiles per dimension
+  int TV;            // tile volume (power of two)
+  int tvb;           // log2(TV)
   long long V, Vh;   // volume, volume / 2^nd
-  long long hs[4];   // strides of the half-extent (y) lexicographic index
+  long long ts[4];   // tile strides (multiples of TV)
   long long ls[4];   // strides of the full lexicographic index
 };
 
 template <int ND>
-__device__ __forceinline__ long long dev_index(const Geom& g, const int* x) {
-  int sub = 0;
+__device__ __forceinline__ long long encode_r(const Geom& g, const int* y) {
   long long r = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d)
+    r += (long long)(y[d] >> g.tb[d]) * g.ts[d] + ((y[d] & ((1 << g.tb[d]) - 1)) << g.lsh[d]);
+  return r;
+}
+template <int ND>
+__device__ __forceinline__ void decode_r(const Geom& g, long long r, int* y) {
+  long long tile = r >> g.tvb;
+  const int loc = (int)(r & (g.TV - 1));
+#pragma unroll
+  for (int d = ND - 1; d >= 0; --d) {
+    const int t = (int)(tile % g.NT[d]);
+    tile /= g.NT[d];
+    y[d] = (t << g.tb[d]) + ((loc >> g.lsh[d]) & ((1 << g.tb[d]) - 1));
+  }
+}
+// coordinates of position r of sublattice sub (index_odd_even)
+template <int ND>
+__device__ __forceinline__ void sub_coords(const Geom& g, int sub, long long r, int* x) {
+  int y[ND];
+  decode_r<ND>(g, r, y);
+#pragma unroll
+  for (int d = 0; d < ND; ++d) x[d] = 2 * y[d] + ((sub >> (ND - 1 - d)) & 1);
+}
+
+template <int ND>
+__device__ __forceinline__ long long dev_index(const Geom& g, const int* x) {
+  int sub = 0, y[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
     sub |= (x[d] & 1) << (ND - 1 - d);
-    r += (long long)(x[d] >> 1) * g.hs[d];
+    y[d] = x[d] >> 1;
   }
-  return (long long)sub * g.Vh + r;
+  return (long long)sub * g.Vh + encode_r<ND>(g, y);
 }
 template <int ND>
 __device__ __forceinline__ long long lex_index(const Geom& g, const int* x) {
@@ -341,13 +379,7 @@ __device__ __forceinline__ long long lex_index(const Geom& g, const int* x) {
 template <int ND>
 __device__ __forceinline__ void dev_coords(const Geom& g, long long s, int* x) {
   const int sub = (int)(s / g.Vh);
-  long long r = s - (long long)sub * g.Vh;
-#pragma unroll
-  for (int d = ND - 1; d >= 0; --d) {
-    const int y = (int)(r % g.H[d]);
-    r /= g.H[d];
-    x[d] = 2 * y + ((sub >> (ND - 1 - d)) & 1);
-  }
+  sub_coords<ND>(g, sub, s - (long long)sub * g.Vh, x);
 }
 // lexicographic index -> coordinates
 template <int ND>
@@ -377,36 +409,30 @@ __device__ __forceinline__ int shiftk(int v, int k, int L) {
 template <int ND>
 __device__ __forceinline__ long long idx_shift(const Geom& g, const int* x, int m1, int s1, int m2,
                                                int s2) {
-  int sub = 0;
-  long long r = 0;
+  int v[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
-    int v = x[d];
+    int w = x[d];
     const int Ld = g.L[d];
-    if (d == m1) v += s1;
-    if (d == m2) v += s2;
-    v = v < 0 ? v + Ld : (v >= Ld ? v - Ld : v);
-    sub |= (v & 1) << (ND - 1 - d);
-    r += (long long)(v >> 1) * g.hs[d];
+    if (d == m1) w += s1;
+    if (d == m2) w += s2;
+    v[d] = w < 0 ? w + Ld : (w >= Ld ? w - Ld : w);
   }
-  return (long long)sub * g.Vh + r;
+  return dev_index<ND>(g, v);
 }
 // the same for arbitrary non-negative shifts k1, k2
 template <int ND>
 __device__ __forceinline__ long long idx_shift_k(const Geom& g, const int* x, int m1, int k1, int m2,
                                                  int k2) {
-  int sub = 0;
-  long long r = 0;
+  int v[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
-    int v = x[d];
-    if (d == m1) v += k1;
-    if (d == m2) v += k2;
-    v %= g.L[d];
-    sub |= (v & 1) << (ND - 1 - d);
-    r += (long long)(v >> 1) * g.hs[d];
+    int w = x[d];
+    if (d == m1) w += k1;
+    if (d == m2) w += k2;
+    v[d] = w % g.L[d];
   }
-  return (long long)sub * g.Vh + r;
+  return dev_index<ND>(g, v);
 }
 
 // ---------------------------------------------------------------------------

@@ -135,31 +135,24 @@ __device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int n
   const unsigned int x = b % 8u, k = b / 8u;
   return (x < rem ? x * (per + 1u) : rem * (per + 1u) + (x - rem) * per) + k;
 }
-// Threads are laid out so that one wave covers 64 consecutive sites of one
-// sublattice (coalesced) and four consecutive logical blocks cover all
-// sublattices of a 64-site chunk, with XCD-contiguous logical blocks.
+// One thread per site: logical block lb covers 256 consecutive positions r
+// (one 4^4 tile when the extents allow it) of sublattice lb % 2^nd, so 2^nd
+// consecutive logical blocks -- kept on one XCD by xcd_block -- cover a tile
+// of every sublattice while each wave reads 64 consecutive sites.
 template <int ND>
 __device__ __forceinline__ bool chunk_site(const Geom &g, unsigned int lb, int *x, long long *s) {
   constexpr int NSUB = 1 << ND;
-  const long long t = (long long)lb * 256 + threadIdx.x;
-  const long long chunk = t / (64 * NSUB);
-  const int sub = (int)((t / 64) % NSUB);
-  const long long r = chunk * 64 + (t & 63);
+  const int sub = (int)(lb % NSUB);
+  const long long r = (long long)(lb / NSUB) * 256 + threadIdx.x;
   if (r >= g.Vh) return false;
-  long long q = r;
-#pragma unroll
-  for (int d = ND - 1; d >= 0; --d) {
-    const int y = (int)(q % g.H[d]);
-    q /= g.H[d];
-    x[d] = 2 * y + ((sub >> (ND - 1 - d)) & 1);
-  }
+  sub_coords<ND>(g, sub, r, x);
   *s = (long long)sub * g.Vh + r;
   return true;
 }
 
 template <int ND>
 __host__ __device__ __forceinline__ long long chunk_threads(const Geom &g) {
-  return ((g.Vh + 63) / 64) * 64 * (1LL << ND);
+  return ((g.Vh + 255) / 256) * 256 * (1LL << ND);
 }
 
 // One thread per link: a block of 64*ND threads holds one 64-site chunk of
@@ -173,13 +166,7 @@ __device__ __forceinline__ bool link_site(const Geom &g, unsigned int lb, int *x
   const long long r = (long long)(lb / NSUB) * 64 + (threadIdx.x & 63);
   *mu = (int)(threadIdx.x >> 6);
   if (r >= g.Vh) return false;
-  long long q = r;
-#pragma unroll
-  for (int d = ND - 1; d >= 0; --d) {
-    const int y = (int)(q % g.H[d]);
-    q /= g.H[d];
-    x[d] = 2 * y + ((sub >> (ND - 1 - d)) & 1);
-  }
+  sub_coords<ND>(g, sub, r, x);
   *s = (long long)sub * g.Vh + r;
   return true;
 }

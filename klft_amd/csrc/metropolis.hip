@@ -32,13 +32,7 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pass(
   unsigned long long n = 0;
   if (t < g.Vh) {
     int x[ND];
-    long long q = t;
-#pragma unroll
-    for (int d = ND - 1; d >= 0; --d) {
-      const int y = (int)(q % g.H[d]);
-      q /= g.H[d];
-      x[d] = 2 * y + ((pass >> (ND - 1 - d)) & 1);  // index_odd_even
-    }
+    sub_coords<ND>(g, pass, t, x);  // index_odd_even
     const long long s = (long long)pass * g.Vh + t;
     const uint32_t site = (uint32_t)lex_index<ND>(g, x);
     if (visit) visit[site] = pass;
@@ -79,27 +73,210 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pass(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Quad variant: four lanes per site.  The proposals (and their accept draws)
+// do not depend on the field, so for hits h0..h0+3 lane q generates the
+// proposal of hit h0+q from its own keyed stream; the quad then steps the
+// Markov chain through the four hits in order, each proposal broadcast from
+// its lane with a DPP quad permute, every lane computing the same decision
+// and the same updated link.  The 2(nd-1) staple terms are split over the
+// lanes (lane q: terms q and q+4) and summed with two commutative DPP
+// exchanges, so all lanes hold the bitwise-identical staple.  Four times the
+// threads of the one-lane kernel and a quarter of its generator work per lane.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL, typename R, int NC>
+__device__ __forceinline__ Mat<R, NC> dpp_mat(const Mat<R, NC> &a) {
+  Mat<R, NC> b;
+#pragma unroll
+  for (int i = 0; i < NC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) b.m[i][j] = mk(dpp<CTRL>(a.m[i][j].re), dpp<CTRL>(a.m[i][j].im));
+  return b;
+}
+// quad_perm broadcast of lane J: [J, J, J, J]
+template <int J, typename R, int NC>
+__device__ __forceinline__ Mat<R, NC> bcast(const Mat<R, NC> &a) {
+  return dpp_mat<J * 0x55>(a);
+}
+template <int J, typename R>
+__device__ __forceinline__ R bcast_s(R v) {
+  return dpp<J * 0x55>(v);
+}
+
+// partial staple of lane q: terms q and q + 4 of [up(nu_0..), low(nu_0..)]
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ Mat<R, NC> staple_part(const Links<R, NC, COMP> &f, const Geom &g,
+                                                  const int *x, int mu, int q) {
+  constexpr int NT = 2 * (ND - 1);
+  Mat<R, NC> t = zero<R, NC>();
+#pragma unroll 1
+  for (int k = q; k < NT; k += 4) {
+    const bool upper = k < ND - 1;
+    const int j = upper ? k : k - (ND - 1);
+    const int nu = j < mu ? j : j + 1;  // j-th direction other than mu
+    if (upper) {
+      const Mat<R, NC> ab = mul_ad(f.load(idx_shift<ND>(g, x, mu, 1, -1, 0), nu),
+                                   f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
+      add_to(t, mul_ad(ab, f.load(dev_index<ND>(g, x), nu)));
+    } else {
+      const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+      const Mat<R, NC> ab = mul_dd(f.load(idx_shift<ND>(g, x, mu, 1, nu, -1), nu), f.load(s_xmnu, mu));
+      add_to(t, mul(ab, f.load(s_xmnu, nu)));
+    }
+  }
+  return t;
+}
+
+template <typename R, int NC>
+__device__ __forceinline__ Mat<R, NC> quad_sum(const Mat<R, NC> &a) {
+  Mat<R, NC> t = a;
+  add_to(t, dpp_mat<0xB1>(t));  // quad_perm [1,0,3,2]
+  add_to(t, dpp_mat<0x4E>(t));  // quad_perm [2,3,0,1]
+  return t;
+}
+
+template <typename R, int NC>
+__device__ __forceinline__ void chain_step(const Mat<R, NC> &r, R u, const Mat<R, NC> &S, R beta_nc,
+                                           Mat<R, NC> &U, Mat<R, NC> &M, R &trM, unsigned long long &n) {
+  const R dS = -beta_nc * (retr_mul(r, M) - trM);
+  bool accept = dS < R(0);
+  if (!accept) accept = u < exp(-dS);
+  if (accept) {
+    U = mul(U, r);
+    reunitarize(U);
+    M = mul(S, U);
+    trM = retrace(M);
+    ++n;
+  }
+}
+
+template <typename R, int NC, bool COMP, int ND, int OCC>
+__global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_quad(
+    Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
+    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  __shared__ unsigned long long red[kMetroBlock / 64];
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long t = ((long long)lb * kMetroBlock + threadIdx.x) >> 2;
+  const int q = threadIdx.x & 3;
+  // a whole quad is either in range or not, so the DPP exchanges never read
+  // an inactive lane
+  const bool active = t < g.Vh;
+  unsigned long long n = 0;
+  if (active) {
+    int x[ND];
+    sub_coords<ND>(g, pass, t, x);
+    const long long s = (long long)pass * g.Vh + t;
+    const uint32_t site = (uint32_t)lex_index<ND>(g, x);
+    if (visit && q == 0) visit[site] = pass;
+#pragma unroll 1
+    for (int mu = 0; mu < ND; ++mu) {
+      const Mat<R, NC> S = quad_sum(staple_part<R, NC, COMP, ND>(f, g, x, mu, q));
+      Mat<R, NC> U = f.load(s, mu);
+      Mat<R, NC> M = mul(S, U);
+      R trM = retrace(M);
+#pragma unroll 1
+      for (int h0 = 0; h0 < nhits; h0 += 4) {
+        Mat<R, NC> r = identity<R, NC>();
+        R u = R(1);
+        if (h0 + q < nhits) {
+          klft_rng::Stream st(seed, klft_rng::TAG_METROPOLIS, site, sweep, (uint32_t)(h0 + q), chain,
+                              (uint32_t)mu);
+          rand_sun(st, delta, r);
+          u = unif<R>(st);  // the accept draw follows the proposal in the hit's stream
+        }
+        chain_step(bcast<0>(r), bcast_s<0>(u), S, beta_nc, U, M, trM, n);
+        if (h0 + 1 < nhits) chain_step(bcast<1>(r), bcast_s<1>(u), S, beta_nc, U, M, trM, n);
+        if (h0 + 2 < nhits) chain_step(bcast<2>(r), bcast_s<2>(u), S, beta_nc, U, M, trM, n);
+        if (h0 + 3 < nhits) chain_step(bcast<3>(r), bcast_s<3>(u), S, beta_nc, U, M, trM, n);
+      }
+      if (nhits > 0 && q == 0) f.store(s, mu, U);
+    }
+    if (q != 0) n = 0;
+  }
+  n = wave_sum_u64(n);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+#pragma unroll
+    for (int i = 0; i < kMetroBlock / 64; ++i) b += red[i];
+    if (b) atomicAdd(nacc, b);
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 namespace klft_internal {
 
-// enqueue one sweep (no synchronisation); acceptance counted into *counter
-int enqueue_sweep(klft_field_t f, const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
-                  unsigned long long *counter, int *visit) {
+// launch passes [p0, p1) of one sweep on the link array `links` with
+// variant 0 (one lane per site) or 1..3 (quad per site, occupancy 1, 2, 4)
+static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p1,
+                         const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
+                         unsigned long long *counter, int *visit) {
   return dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
-    const unsigned nb = blocks_for(f->g.Vh, kMetroBlock);
+    Links<R, NC, COMP> l;
+    l.p = reinterpret_cast<cx<R> *>(links);
+    l.V = f->g.V;
     const R beta_nc = (R)(p->beta / (double)NC);
-    for (int pass = 0; pass < (1 << ND); ++pass) {
-      hipLaunchKernelGGL((k_metropolis_pass<R, NC, COMP, ND>), dim3(nb), dim3(kMetroBlock), 0, stream(),
-                         links_of<R, NC, COMP>(f), f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
-                         (uint32_t)sweep, chain, counter, visit);
+    const int lanes = variant >= 1 ? 4 : 1;
+    const unsigned nb = blocks_for(f->g.Vh * lanes, kMetroBlock);
+    for (int pass = p0; pass < p1; ++pass) {
+      if (variant >= 1)
+        with_occ(variant - 1, [&](auto occ) -> int {
+          constexpr int OCC = decltype(occ)::value;
+          hipLaunchKernelGGL((k_metropolis_quad<R, NC, COMP, ND, OCC>), dim3(nb), dim3(kMetroBlock), 0,
+                             stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
+                             (uint32_t)sweep, chain, counter, visit);
+          return KLFT_OK;
+        });
+      else
+        hipLaunchKernelGGL((k_metropolis_pass<R, NC, COMP, ND>), dim3(nb), dim3(kMetroBlock), 0, stream(), l,
+                           f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed, (uint32_t)sweep, chain,
+                           counter, visit);
     }
     KLFT_HIP(hipGetLastError());
     return KLFT_OK;
   });
+}
+
+// sweep variant for this field; tuning times one pass of each variant on a
+// scratch copy of the links, so the live field is never updated by it
+int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
+  void *scratch = nullptr;
+  // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4 waves/SIMD
+  const int v = pick_variant("metropolis", f, 4, 1, [&](int var) -> int {
+    if (!scratch) {
+      KLFT_HIP(hipMalloc(&scratch, f->link_bytes));
+      KLFT_HIP(hipMemcpyAsync(scratch, f->links, f->link_bytes, hipMemcpyDeviceToDevice, stream()));
+    }
+    return launch_passes(f, scratch, var, 0, 1, p, 0, 0, f->counter + 7, nullptr);
+  });
+  if (scratch) {
+    (void)hipStreamSynchronize(stream());
+    (void)hipFree(scratch);
+  }
+  return v;
+}
+
+// enqueue one sweep (no synchronisation); acceptance counted into *counter
+int enqueue_sweep(klft_field_t f, const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
+                  unsigned long long *counter, int *visit) {
+  const int v = metropolis_variant(f, p);
+  return launch_passes(f, f->links, v, 0, 1 << f->nd, p, sweep, chain, counter, visit);
 }
 
 }  // namespace klft_internal
