@@ -202,6 +202,13 @@ int klft_tune_cache_read(const char *path);
 int klft_tune_cache_write(const char *path);
 int klft_tune_lookup(const char *key, int *variant);
 
+/* ---- kernel timing: HIP events on the library stream around each launch of
+ * the named kernel ("metropolis_pass", "force", "update_gauge", "plaquette",
+ * "staple_field", "wilson"); the reference prints Kokkos::Timer values ---- */
+int klft_profile(int on); /* enable/disable; clears recorded events */
+/* sum of durations and number of launches recorded since the last read */
+int klft_profile_read(const char *name, double *total_ms, int64_t *count);
+
 /* ---- drop-in driver (klft::Metropolis, lib/Metropolis.cpp:36-166) ------------------ */
 /* Reads the YAML input, runs the sweeps and writes the observable files
  * exactly as the reference's metropolis binary.  Returns 0 or -1. */

@@ -111,6 +111,8 @@ _SIGS = {
     "klft_tune_cache_read": (C.c_int, [C.c_char_p]),
     "klft_tune_cache_write": (C.c_int, [C.c_char_p]),
     "klft_tune_lookup": (C.c_int, [C.c_char_p, _ip]),
+    "klft_profile": (C.c_int, [C.c_int]),
+    "klft_profile_read": (C.c_int, [C.c_char_p, _dp, _i64p]),
 }
 
 EXPORTED = []
@@ -363,3 +365,28 @@ def gauge_action(g: GaugeField, beta: float) -> float:
     o = C.c_double()
     _check(lib.klft_gauge_action(g.h, beta, C.byref(o)))
     return o.value
+
+
+def profile(on: bool = True) -> None:
+    """Enable kernel timing (HIP events on the library stream)."""
+    _check(lib.klft_profile(int(on)))
+
+
+def profile_read(name: str):
+    """(total_ms, launches) recorded for kernel `name` since the last read."""
+    ms = C.c_double()
+    n = C.c_int64()
+    _check(lib.klft_profile_read(name.encode(), C.byref(ms), C.byref(n)))
+    return ms.value, n.value
+
+
+def tuning(on: bool = True) -> None:
+    _check(lib.klft_tuning(int(on)))
+
+
+def tune_cache_read(path: str) -> None:
+    _check(lib.klft_tune_cache_read(str(path).encode()))
+
+
+def tune_cache_write(path: str) -> None:
+    _check(lib.klft_tune_cache_write(str(path).encode()))

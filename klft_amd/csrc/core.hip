@@ -520,6 +520,7 @@ static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int 
     if (nb > (unsigned)kMaxPartials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
     return with_occ(variant, [&](auto occ) -> int {
       constexpr int OCC = decltype(occ)::value;
+      ProfScope prof("plaquette");
       hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, OCC>), dim3(nb), dim3(256), 0, stream(),
                          links_of<R, NC, COMP>(f), f->g, f->partials, f->ticket, f->result + slot,
                          dev_per_site);
@@ -595,6 +596,7 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
       const unsigned nb = (unsigned)link_blocks<ND>(f->g);
       return ki::with_occ(variant, [&](auto occ) -> int {
         constexpr int OCC = decltype(occ)::value;
+        ki::ProfScope prof("staple_field");
         hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND, OCC>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
                            ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
         KLFT_HIP(hipGetLastError());
@@ -737,6 +739,7 @@ int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, 
   }
   int rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
+    ki::ProfScope prof("wilson");
     hipLaunchKernelGGL((k_wilson<R, NC, COMP, ND>), dim3(nbx, nplanes), dim3(block),
                        block * sizeof(Mat<R, NC>), ki::stream(), ki::links_of<R, NC, COMP>(f), f->g,
                        nplanes > 1 ? -1 : mu0, nu, C, nL, dLs, Tmax, dpart, dps, ps_li, ps_T);

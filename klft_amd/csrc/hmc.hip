@@ -198,6 +198,7 @@ int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int vari
     if (nb > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
     return ki::with_occ(variant, [&](auto occ) -> int {
       constexpr int OCC = decltype(occ)::value;
+      ki::ProfScope prof("force");
       hipLaunchKernelGGL((k_force<R, NC, COMP, ND, OCC>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
                          ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC),
                          (R)dtau, Fout, f->partials, f->ticket,
@@ -219,6 +220,7 @@ int enqueue_gauge(klft_hmc_t h, double dtau) {
   klft_field_t f = h->f;
   return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
+    ki::ProfScope prof("update_gauge");
     hipLaunchKernelGGL((k_update_gauge<R, NC, COMP, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256),
                        0, ki::stream(), ki::links_of<R, NC, COMP>(f), (const R *)h->P, f->g, (R)dtau);
     KLFT_HIP(hipGetLastError());

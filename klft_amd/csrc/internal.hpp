@@ -57,6 +57,27 @@ int sync_stream();
 template <int V>
 using ic = std::integral_constant<int, V>;
 
+// ---- kernel timing (tuner.hip): HIP events on the library stream around
+// launches, enabled with klft_profile(1); read back with klft_profile_read ----
+bool prof_enabled();
+void prof_record(const char *name, hipEvent_t start, hipEvent_t stop);
+struct ProfScope {
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  explicit ProfScope(const char *n) : name(n) {
+    if (prof_enabled() && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, stream());
+    else
+      a = b = nullptr;
+  }
+  ~ProfScope() {
+    if (a) {
+      (void)hipEventRecord(b, stream());
+      prof_record(name, a, b);
+    }
+  }
+};
+
 // ---- autotuner (tuner.hip) ----
 bool tuning_enabled();
 std::string tune_key(const char *kernel, const klft_field_s *f);

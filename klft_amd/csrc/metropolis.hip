@@ -235,6 +235,7 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
     const int lanes = variant >= 1 ? 4 : 1;
     const unsigned nb = blocks_for(f->g.Vh * lanes, kMetroBlock);
     for (int pass = p0; pass < p1; ++pass) {
+      ProfScope prof("metropolis_pass");
       if (variant >= 1)
         with_occ(variant - 1, [&](auto occ) -> int {
           constexpr int OCC = decltype(occ)::value;

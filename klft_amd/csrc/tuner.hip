@@ -21,6 +21,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -110,9 +111,62 @@ int pick_variant(const char *kernel, const klft_field_s *f, int nvar, int dflt,
   return best;
 }
 
+// ---- kernel timing ----
+struct ProfRec {
+  std::string name;
+  hipEvent_t a, b;
+};
+static std::mutex g_pmu;
+static std::vector<ProfRec> g_prof;
+static bool g_prof_on = false;
+
+bool prof_enabled() { return g_prof_on; }
+
+void prof_record(const char *name, hipEvent_t start, hipEvent_t stop) {
+  std::lock_guard<std::mutex> lk(g_pmu);
+  g_prof.push_back(ProfRec{name, start, stop});
+}
+
 }  // namespace klft_internal
 
 namespace ki = klft_internal;
+
+extern "C" int klft_profile(int on) {
+  std::lock_guard<std::mutex> lk(ki::g_pmu);
+  for (auto &r : ki::g_prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  ki::g_prof.clear();
+  ki::g_prof_on = on != 0;
+  return KLFT_OK;
+}
+
+extern "C" int klft_profile_read(const char *name, double *total_ms, int64_t *count) {
+  if (!name || !total_ms || !count) return ki::set_error(KLFT_E_ARG, "null argument");
+  KLFT_HIP(hipStreamSynchronize(ki::stream()));
+  std::lock_guard<std::mutex> lk(ki::g_pmu);
+  double ms = 0.0;
+  int64_t n = 0;
+  std::vector<ki::ProfRec> keep;
+  for (auto &r : ki::g_prof) {
+    if (r.name == name) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+        ms += t;
+        ++n;
+      }
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    } else {
+      keep.push_back(r);
+    }
+  }
+  ki::g_prof.swap(keep);
+  *total_ms = ms;
+  *count = n;
+  return KLFT_OK;
+}
 
 extern "C" int klft_tuning(int on) {
   ki::g_tuning = on ? 1 : 0;

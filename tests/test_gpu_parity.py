@@ -182,3 +182,28 @@ def test_fp32_plaquette_within_1e6(nc, storage):
     g = klft.GaugeField(nd, nc, dims, storage).set_random_algebra(0.5, 13).reunitarize()
     got = klft.GaugePlaquette(g)
     assert abs(got - ref) < 1e-6
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("nc,beta,storage", [(1, 2.0, "f64"), (2, 2.3, "f64"), (3, 5.5, "f64"),
+                                             (3, 5.5, "f32"), (2, 2.3, "f32_rows")])
+def test_metropolis_variants_agree(monkeypatch, variant, nc, beta, storage):
+    """Every sweep variant (one lane per site, quad per site at three occupancy
+    targets) makes the same decisions: FP64 against the oracle, FP32 against
+    variant 0."""
+    nd, dims = 4, (4, 6, 4, 4)
+    U = orc.random_algebra(nd, nc, dims, 0.3, 19)
+    p = klft.MetropolisParams(nHits=7, beta=beta, delta=0.3, seed=99)
+    monkeypatch.setenv("KLFT_VARIANT", f"metropolis={variant}")
+    g = klft.GaugeField(nd, nc, dims, storage).upload(U)
+    n_g = [klft.sweep_Metropolis(g, p, s, with_count=True)[1] for s in range(2)]
+    if storage == "f64":
+        Uo = U.copy()
+        n_o = [orc.metropolis_sweep(Uo, nd, nc, dims, 7, beta, 0.3, 99, s)[1] for s in range(2)]
+        assert n_g == n_o
+        np.testing.assert_allclose(g.download(), Uo, rtol=0, atol=1e-11)
+    else:
+        monkeypatch.setenv("KLFT_VARIANT", "metropolis=0")
+        g0 = klft.GaugeField(nd, nc, dims, storage).upload(U)
+        n_0 = [klft.sweep_Metropolis(g0, p, s, with_count=True)[1] for s in range(2)]
+        assert abs(n_g[0] - n_0[0]) <= 2 and abs(n_g[1] - n_0[1]) <= 4
