@@ -390,3 +390,89 @@ def tune_cache_read(path: str) -> None:
 
 def tune_cache_write(path: str) -> None:
     _check(lib.klft_tune_cache_write(str(path).encode()))
+
+
+class Batch:
+    """Many independent 4D chains updated by one launch per sublattice pass."""
+
+    def __init__(self, chains, storage="f32"):
+        """chains: list of (nc, (L0, L1, L2, L3), beta)."""
+        if isinstance(storage, str):
+            storage = STORAGES[storage]
+        arr = (ChainDescC * len(chains))()
+        for i, (nc, dims, beta) in enumerate(chains):
+            arr[i].nc = nc
+            for d in range(4):
+                arr[i].dims[d] = dims[d]
+            arr[i].beta = beta
+        self.chains = [(nc, tuple(dims), beta) for nc, dims, beta in chains]
+        h = _p()
+        _check(lib.klft_batch_create(len(chains), arr, storage, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.klft_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_identity(self):
+        _check(lib.klft_batch_set_identity(self.h))
+
+    def sweep(self, nhits, delta, seed, sweep):
+        acc = np.zeros(len(self.chains))
+        _check(lib.klft_batch_sweep(self.h, nhits, delta, seed, sweep, _dptr(acc)))
+        return acc
+
+    def run(self, nhits, delta, seed, sweep0, n):
+        _check(lib.klft_batch_run(self.h, nhits, delta, seed, sweep0, n))
+
+    def plaquette(self):
+        out = np.zeros(len(self.chains))
+        _check(lib.klft_batch_plaquette(self.h, _dptr(out)))
+        return out
+
+    def download(self, i):
+        nc, dims, _ = self.chains[i]
+        a = np.empty((*dims, 4, nc, nc), dtype=np.complex128)
+        _check(lib.klft_batch_download(self.h, i, a.ctypes.data_as(_dp)))
+        return a
+
+    def upload(self, i, U):
+        a = np.ascontiguousarray(U, dtype=np.complex128)
+        _check(lib.klft_batch_upload(self.h, i, a.ctypes.data_as(_dp)))
+
+
+class PTBC(Batch):
+    """Replica set with a defect whose boundary links carry coupling c[r]."""
+
+    def __init__(self, nreplicas, nc, dims, beta, defect_length, defect_pos, c, storage="f32"):
+        if isinstance(storage, str):
+            storage = STORAGES[storage]
+        self.chains = [(nc, tuple(dims), beta)] * nreplicas
+        cv = np.ascontiguousarray(c, dtype=np.float64)
+        h = _p()
+        _check(lib.klft_ptbc_create(nreplicas, nc, _iarr(dims), storage, beta, defect_length, _iarr(defect_pos),
+                                    _dptr(cv), C.byref(h)))
+        self.h = h
+        self.c = cv
+
+    def step(self, nhits, delta, seed, step):
+        sw = np.zeros(max(1, len(self.chains) - 1), dtype=np.int32)
+        _check(lib.klft_ptbc_step(self.h, nhits, delta, seed, step, sw.ctypes.data_as(_ip)))
+        return sw[:len(self.chains) - 1]
+
+    def labels(self):
+        out = np.zeros(len(self.chains), dtype=np.int32)
+        _check(lib.klft_ptbc_labels(self.h, out.ctypes.data_as(_ip)))
+        return out
+
+    def action(self, replica, coupling):
+        o = C.c_double()
+        _check(lib.klft_ptbc_action(self.h, replica, coupling, C.byref(o)))
+        return o.value

@@ -32,8 +32,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INCLUDE}"
           "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
 
 HIP_SOURCES = ["core.hip", "metropolis.hip", "hmc.hip", "tuner.hip", "batch.hip"]
-HOST_SOURCES = ["host/yaml_lite.cpp", "host/observables.cpp", "host/driver.cpp",
-                "host/tuner.cpp"]
+HOST_SOURCES = ["host/driver.cpp"]
 TOOLS = {"metropolis": "tools/metropolis_main.cpp", "hmc": "tools/hmc_main.cpp",
          "plaq_bench": "tools/plaq_bench.cpp", "wloop_temporal_bench": "tools/wloop_bench.cpp",
          "metropolis_bench": "tools/metropolis_bench.cpp"}

@@ -1,0 +1,554 @@
+// Batched Metropolis for many independent 4D chains and PTBC replica sets.
+//
+// One launch per sublattice pass updates every chain: the grid is a work
+// table of 256-site blocks, block b belonging to chain block_chain[b], so
+// chains of uneven volume and of different groups (U(1), SU(2), SU(3): the
+// group is uniform per block) and different beta share one dispatch.  Each
+// chain draws from its own keyed streams (chain index in the Philox counter).
+// A sweep of the whole batch -- counter reset, 16 pass launches and, for
+// replicas, the defect action sums and the swap kernel -- is captured once in
+// a HIP graph and replayed; the sweep / step index lives in device memory and
+// the graph's last node advances it.
+//
+// PTBC (PTBCGaugeField.hpp:30-279): replica r's links on the defect (mu = 0,
+// x0 = 0, x_k in [p_k, p_k + Ld)) are scaled by its coupling c.  Swaps of
+// neighbouring couplings (c_k, c_k+1) between configurations a, b use
+//   dS = -(beta/Nc) [(c_k - c_k+1)(A1_b - A1_a) + (c_k^2 - c_k+1^2)(A2_b - A2_a)],
+// with A1 / A2 the FP64 sums of Re Tr U_P over plaquettes through one / two
+// defect links (unscaled): only those plaquettes depend on c.  The labels
+// (which configuration carries which coupling) are exchanged, never links.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.hpp"
+#include "metro.hpp"
+
+using namespace kd;
+namespace ki = klft_internal;
+
+namespace {
+
+struct ChainDev {
+  void *links;
+  Geom g;
+  int nc;
+  double beta;
+  int first_block;   // pass grid (positions of one sublattice)
+  int first_pblock;  // plaquette grid (all sites)
+};
+
+constexpr int kB = 256;
+
+template <typename R, bool COMP, bool DEFECT>
+__global__ __launch_bounds__(kB) void k_batch_pass(const ChainDev *__restrict__ chains,
+                                                   const int *__restrict__ block_chain, int pass, int nhits,
+                                                   R delta, uint64_t seed, const uint32_t *__restrict__ d_sweep,
+                                                   unsigned long long *counters, Defect d,
+                                                   const double *__restrict__ cval) {
+  __shared__ unsigned long long red[kB / 64];
+  const int c = block_chain[blockIdx.x];
+  const ChainDev ch = chains[c];
+  const long long t = (long long)(blockIdx.x - ch.first_block) * kB + threadIdx.x;
+  const uint32_t sweep = *d_sweep;
+  unsigned long long n = 0;
+  if (t < ch.g.Vh) {
+    const R cc = DEFECT ? (R)cval[c] : R(1);
+    switch (ch.nc) {
+      case 1: {
+        Links<R, 1, false> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V};
+        n = metro_site<R, 1, false, 4, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 1.0), delta, seed, sweep,
+                                               (uint32_t)c, d, cc, nullptr);
+        break;
+      }
+      case 2: {
+        Links<R, 2, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V};
+        n = metro_site<R, 2, COMP, 4, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 2.0), delta, seed, sweep,
+                                              (uint32_t)c, d, cc, nullptr);
+        break;
+      }
+      default: {
+        Links<R, 3, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V};
+        n = metro_site<R, 3, COMP, 4, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 3.0), delta, seed, sweep,
+                                              (uint32_t)c, d, cc, nullptr);
+        break;
+      }
+    }
+  }
+  n = wave_sum_u64(n);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+#pragma unroll
+    for (int i = 0; i < kB / 64; ++i) b += red[i];
+    if (b) atomicAdd(&counters[c], b);
+  }
+}
+
+template <typename R, bool COMP>
+__global__ __launch_bounds__(kB) void k_batch_plaq(const ChainDev *__restrict__ chains,
+                                                   const int *__restrict__ pblock_chain, double *partials) {
+  __shared__ double lds[kB / 64];
+  const int c = pblock_chain[blockIdx.x];
+  const ChainDev ch = chains[c];
+  const long long s = (long long)(blockIdx.x - ch.first_pblock) * kB + threadIdx.x;
+  double v = 0.0;
+  if (s < ch.g.V) {
+    int x[4];
+    dev_coords<4>(ch.g, s, x);
+    switch (ch.nc) {
+      case 1: v = plaq_at<R, 1, false, 4>(Links<R, 1, false>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V}, ch.g, x, s); break;
+      case 2: v = plaq_at<R, 2, COMP, 4>(Links<R, 2, COMP>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V}, ch.g, x, s); break;
+      default: v = plaq_at<R, 3, COMP, 4>(Links<R, 3, COMP>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V}, ch.g, x, s); break;
+    }
+  }
+  const double bt = block_sum<kB>(v, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bt;
+}
+
+// per-chain sums of the plaquette partials, in block order
+__global__ __launch_bounds__(kB) void k_batch_reduce(const ChainDev *__restrict__ chains, int nchains, int total,
+                                                     const double *partials, double *out) {
+  __shared__ double lds[kB / 64];
+  const int c = blockIdx.x;
+  const int b0 = chains[c].first_pblock;
+  const int b1 = c + 1 < nchains ? chains[c + 1].first_pblock : total;
+  double v = 0.0;
+  for (int i = b0 + (int)threadIdx.x; i < b1; i += kB) v += partials[i];
+  v = block_sum<kB>(v, lds);
+  if (threadIdx.x == 0) out[c] = v;
+}
+
+// A1 / A2 of every replica: one thread per replica visits the (0, nu)
+// plaquettes whose mu = 0 links touch the defect, in a fixed order
+template <typename R, int NC, bool COMP>
+__global__ void k_defect_sums(const ChainDev *__restrict__ chains, int n, Defect d, double *A) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const ChainDev ch = chains[c];
+  const Geom &g = ch.g;
+  Links<R, NC, COMP> f{reinterpret_cast<cx<R> *>(ch.links), g.V};
+  double a1 = 0.0, a2 = 0.0;
+  for (int nu = 1; nu < 4; ++nu) {
+    // base sites y: y0 = 0, y_nu in [p_nu - 1, p_nu + Ld), the others in [p_k, p_k + Ld)
+    int ext[4] = {1, d.len, d.len, d.len};
+    ext[nu] = d.len + 1;
+    for (int i1 = 0; i1 < ext[1]; ++i1)
+      for (int i2 = 0; i2 < ext[2]; ++i2)
+        for (int i3 = 0; i3 < ext[3]; ++i3) {
+          int y[4] = {0, d.pos[0] + i1, d.pos[1] + i2, d.pos[2] + i3};
+          y[nu] -= 1;
+          for (int k = 1; k < 4; ++k) y[k] = ((y[k] % g.L[k]) + g.L[k]) % g.L[k];
+          int yn[4], y0[4];
+          shifted<4>(g, y, nu, 1, -1, 0, yn);
+          shifted<4>(g, y, 0, 1, -1, 0, y0);
+          const int k = (int)on_defect<4>(g, d, y, 0) + (int)on_defect<4>(g, d, yn, 0);
+          if (k == 0) continue;
+          const Mat<R, NC> lmu = mul(f.load(dev_index<4>(g, y), 0), f.load(dev_index<4>(g, y0), nu));
+          const Mat<R, NC> lnu = mul(f.load(dev_index<4>(g, y), nu), f.load(dev_index<4>(g, yn), 0));
+          const double p = (double)retr_mul_ad(lmu, lnu);
+          if (k == 1)
+            a1 += p;
+          else
+            a2 += p;
+        }
+  }
+  A[2 * c] = a1;
+  A[2 * c + 1] = a2;
+}
+
+// sequential nearest-neighbour swap proposals of the coupling labels
+__global__ void k_swap(int n, int nc, double beta, const double *__restrict__ cvals, const double *__restrict__ A,
+                       int *conf_of, double *cval, int *swaps, uint64_t seed, const uint32_t *d_step) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t step = *d_step;
+  for (int k = 0; k + 1 < n; ++k) {
+    const int a = conf_of[k], b = conf_of[k + 1];
+    const double ck = cvals[k], ck1 = cvals[k + 1];
+    const double dS = -(beta / (double)nc) *
+                      ((ck - ck1) * (A[2 * b] - A[2 * a]) + (ck * ck - ck1 * ck1) * (A[2 * b + 1] - A[2 * a + 1]));
+    klft_rng::Stream st(seed, klft_rng::TAG_SWAP, (uint32_t)k, step, 0, 0, 0);
+    const double u = st.u01();
+    const bool acc = dS <= 0.0 || u < exp(-dS);
+    swaps[k] = acc ? 1 : 0;
+    if (acc) {
+      conf_of[k] = b;
+      conf_of[k + 1] = a;
+    }
+  }
+  for (int k = 0; k < n; ++k) cval[conf_of[k]] = cvals[k];
+}
+
+__global__ void k_zero_counters(unsigned long long *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+__global__ void k_advance(uint32_t *p) {
+  if (threadIdx.x == 0) *p += 1u;
+}
+
+}  // namespace
+
+struct klft_batch_s {
+  int n = 0, storage = 0;
+  std::vector<klft_field_t> fields;
+  std::vector<double> beta;
+  ChainDev *d_chains = nullptr;
+  int *d_block_chain = nullptr, *d_pblock_chain = nullptr;
+  int nblocks = 0, npblocks = 0;
+  double *d_partials = nullptr, *d_plaq = nullptr;
+  unsigned long long *d_counters = nullptr;
+  uint32_t *d_sweep = nullptr;
+  // graph of one sweep / step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int g_nhits = -1;
+  double g_delta = 0.0;
+  uint64_t g_seed = 0;
+  // PTBC
+  bool ptbc = false;
+  int nc = 0;
+  Defect defect{};
+  std::vector<double> cvals;
+  double *d_cvals = nullptr, *d_cval = nullptr, *d_A = nullptr;
+  int *d_conf_of = nullptr, *d_swaps = nullptr;
+};
+
+namespace {
+
+int check_batch(klft_batch_t b) {
+  if (!b || !b->d_chains) return ki::set_error(KLFT_E_ARG, "null or destroyed batch handle");
+  return KLFT_OK;
+}
+
+void free_batch(klft_batch_t b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto f : b->fields) klft_field_destroy(f);
+  void *ptrs[] = {b->d_chains, b->d_block_chain, b->d_pblock_chain, b->d_partials, b->d_plaq, b->d_counters,
+                  b->d_sweep, b->d_cvals, b->d_cval, b->d_A, b->d_conf_of, b->d_swaps};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  b->d_chains = nullptr;
+}
+
+int build_tables(klft_batch_t b) {
+  std::vector<ChainDev> ch(b->n);
+  std::vector<int> bc, pbc;
+  for (int c = 0; c < b->n; ++c) {
+    klft_field_t f = b->fields[c];
+    ch[c].links = f->links;
+    ch[c].g = f->g;
+    ch[c].nc = f->nc;
+    ch[c].beta = b->beta[c];
+    ch[c].first_block = (int)bc.size();
+    ch[c].first_pblock = (int)pbc.size();
+    const long long nb = (f->g.Vh + kB - 1) / kB, npb = (f->g.V + kB - 1) / kB;
+    bc.insert(bc.end(), nb, c);
+    pbc.insert(pbc.end(), npb, c);
+  }
+  b->nblocks = (int)bc.size();
+  b->npblocks = (int)pbc.size();
+  KLFT_HIP(hipMalloc(&b->d_chains, sizeof(ChainDev) * b->n));
+  KLFT_HIP(hipMalloc(&b->d_block_chain, sizeof(int) * bc.size()));
+  KLFT_HIP(hipMalloc(&b->d_pblock_chain, sizeof(int) * pbc.size()));
+  KLFT_HIP(hipMalloc(&b->d_partials, sizeof(double) * pbc.size()));
+  KLFT_HIP(hipMalloc(&b->d_plaq, sizeof(double) * b->n));
+  KLFT_HIP(hipMalloc(&b->d_counters, sizeof(unsigned long long) * b->n));
+  KLFT_HIP(hipMalloc(&b->d_sweep, sizeof(uint32_t)));
+  KLFT_HIP(hipMemcpy(b->d_chains, ch.data(), sizeof(ChainDev) * b->n, hipMemcpyHostToDevice));
+  KLFT_HIP(hipMemcpy(b->d_block_chain, bc.data(), sizeof(int) * bc.size(), hipMemcpyHostToDevice));
+  KLFT_HIP(hipMemcpy(b->d_pblock_chain, pbc.data(), sizeof(int) * pbc.size(), hipMemcpyHostToDevice));
+  KLFT_HIP(hipMemset(b->d_counters, 0, sizeof(unsigned long long) * b->n));
+  KLFT_HIP(hipMemset(b->d_sweep, 0, sizeof(uint32_t)));
+  return KLFT_OK;
+}
+
+// enqueue one sweep (and, for replicas, the swap pass); no host sync
+int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, ki::stream(), b->d_counters, b->n);
+  auto passes = [&](auto rt, auto ct) -> int {
+    using R = typename decltype(rt)::type;
+    constexpr bool COMP = decltype(ct)::value != 0;
+    for (int pass = 0; pass < 16; ++pass) {
+      if (b->ptbc)
+        hipLaunchKernelGGL((k_batch_pass<R, COMP, true>), dim3(b->nblocks), dim3(kB), 0, ki::stream(),
+                           b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters,
+                           b->defect, b->d_cval);
+      else
+        hipLaunchKernelGGL((k_batch_pass<R, COMP, false>), dim3(b->nblocks), dim3(kB), 0, ki::stream(),
+                           b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters,
+                           b->defect, b->d_cval);
+    }
+    if (b->ptbc) {
+      switch (b->nc) {
+        case 1: hipLaunchKernelGGL((k_defect_sums<R, 1, false>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
+        case 2: hipLaunchKernelGGL((k_defect_sums<R, 2, COMP>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
+        default: hipLaunchKernelGGL((k_defect_sums<R, 3, COMP>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
+      }
+      hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, ki::stream(), b->n, b->nc, b->beta[0], b->d_cvals, b->d_A,
+                         b->d_conf_of, b->d_cval, b->d_swaps, seed, b->d_sweep);
+    }
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  };
+  int rc;
+  switch (b->storage) {
+    case KLFT_STORAGE_F64: rc = passes(ki::tag<double>{}, ki::ic<0>{}); break;
+    case KLFT_STORAGE_F32: rc = passes(ki::tag<float>{}, ki::ic<0>{}); break;
+    default: rc = passes(ki::tag<float>{}, ki::ic<1>{}); break;
+  }
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ki::stream(), b->d_sweep);
+  KLFT_HIP(hipGetLastError());
+  return KLFT_OK;
+}
+
+int ensure_graph(klft_batch_t b, int nhits, double delta, uint64_t seed) {
+  if (b->exec && b->g_nhits == nhits && b->g_delta == delta && b->g_seed == seed) return KLFT_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  KLFT_HIP(hipStreamBeginCapture(ki::stream(), hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_step(b, nhits, delta, seed);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ki::stream(), &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return ki::hip_fail(e, "graph capture");
+  b->graph = g;
+  KLFT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->g_nhits = nhits;
+  b->g_delta = delta;
+  b->g_seed = seed;
+  return KLFT_OK;
+}
+
+double acceptance(klft_field_t f, int nhits, unsigned long long n) {
+  double norm = 1.0;
+  for (int d = 0; d < f->nd; ++d) norm *= (double)f->dims[d];
+  norm *= (double)f->nd;
+  norm *= (double)nhits;
+  return (double)n / norm;
+}
+
+}  // namespace
+
+extern "C" int klft_batch_create(int nchains, const klft_chain_desc *chains, int storage, klft_batch_t *out) {
+  if (!out || !chains || nchains < 1) return ki::set_error(KLFT_E_ARG, "invalid batch arguments");
+  if (nchains > 1024) return ki::set_error(KLFT_E_ARG, "at most 1024 chains per batch");
+  *out = nullptr;
+  klft_batch_s *b = new klft_batch_s();
+  b->n = nchains;
+  b->storage = storage;
+  for (int c = 0; c < nchains; ++c) {
+    klft_field_t f = nullptr;
+    const int rc = klft_field_create(4, chains[c].nc, chains[c].dims, storage, &f);
+    if (rc) {
+      free_batch(b);
+      delete b;
+      return rc;
+    }
+    b->fields.push_back(f);
+    b->beta.push_back(chains[c].beta);
+  }
+  const int rc = build_tables(b);
+  if (rc) {
+    free_batch(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return KLFT_OK;
+}
+
+extern "C" int klft_batch_destroy(klft_batch_t b) {
+  if (!b) return KLFT_OK;
+  (void)ki::sync_stream();
+  free_batch(b);
+  delete b;
+  return KLFT_OK;
+}
+
+extern "C" int klft_batch_set_identity(klft_batch_t b) {
+  int rc = check_batch(b);
+  for (int c = 0; c < b->n && !rc; ++c) rc = klft_field_set_identity(b->fields[c]);
+  return rc;
+}
+
+extern "C" int klft_batch_sweep(klft_batch_t b, int nhits, double delta, uint64_t seed, uint64_t sweep, double *acc) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (nhits < 0) return ki::set_error(KLFT_E_ARG, "nHits must be >= 0");
+  const uint32_t s = (uint32_t)sweep;
+  KLFT_HIP(hipMemcpyAsync(b->d_sweep, &s, sizeof s, hipMemcpyHostToDevice, ki::stream()));
+  rc = enqueue_step(b, nhits, delta, seed);
+  if (rc) return rc;
+  std::vector<unsigned long long> n(b->n);
+  KLFT_HIP(hipMemcpyAsync(n.data(), b->d_counters, sizeof(unsigned long long) * b->n, hipMemcpyDeviceToHost,
+                          ki::stream()));
+  KLFT_HIP(hipStreamSynchronize(ki::stream()));
+  if (acc)
+    for (int c = 0; c < b->n; ++c) acc[c] = acceptance(b->fields[c], nhits, n[c]);
+  return KLFT_OK;
+}
+
+extern "C" int klft_batch_run(klft_batch_t b, int nhits, double delta, uint64_t seed, uint64_t sweep0, int n) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (nhits < 0 || n < 0) return ki::set_error(KLFT_E_ARG, "invalid sweep count");
+  rc = ensure_graph(b, nhits, delta, seed);
+  if (rc) return rc;
+  const uint32_t s = (uint32_t)sweep0;
+  KLFT_HIP(hipMemcpyAsync(b->d_sweep, &s, sizeof s, hipMemcpyHostToDevice, ki::stream()));
+  for (int i = 0; i < n; ++i) KLFT_HIP(hipGraphLaunch(b->exec, ki::stream()));
+  return ki::sync_stream();
+}
+
+extern "C" int klft_batch_plaquette(klft_batch_t b, double *out) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!out) return ki::set_error(KLFT_E_ARG, "null output");
+  auto launch = [&](auto rt, auto ct) -> int {
+    using R = typename decltype(rt)::type;
+    constexpr bool COMP = decltype(ct)::value != 0;
+    hipLaunchKernelGGL((k_batch_plaq<R, COMP>), dim3(b->npblocks), dim3(kB), 0, ki::stream(), b->d_chains,
+                       b->d_pblock_chain, b->d_partials);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  };
+  switch (b->storage) {
+    case KLFT_STORAGE_F64: rc = launch(ki::tag<double>{}, ki::ic<0>{}); break;
+    case KLFT_STORAGE_F32: rc = launch(ki::tag<float>{}, ki::ic<0>{}); break;
+    default: rc = launch(ki::tag<float>{}, ki::ic<1>{}); break;
+  }
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_batch_reduce, dim3(b->n), dim3(kB), 0, ki::stream(), b->d_chains, b->n, b->npblocks,
+                     b->d_partials, b->d_plaq);
+  KLFT_HIP(hipGetLastError());
+  std::vector<double> raw(b->n);
+  KLFT_HIP(hipMemcpyAsync(raw.data(), b->d_plaq, sizeof(double) * b->n, hipMemcpyDeviceToHost, ki::stream()));
+  KLFT_HIP(hipStreamSynchronize(ki::stream()));
+  for (int c = 0; c < b->n; ++c) {  // GaugePlaquette.hpp:125-132
+    klft_field_t f = b->fields[c];
+    double norm = 1.0;
+    for (int d = 0; d < f->nd; ++d) norm *= (double)f->dims[d];
+    norm *= (double)((f->nd * (f->nd - 1) / 2) * f->nc);
+    out[c] = raw[c] / norm;
+  }
+  return KLFT_OK;
+}
+
+extern "C" int klft_batch_download(klft_batch_t b, int chain, double *host) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (chain < 0 || chain >= b->n) return ki::set_error(KLFT_E_ARG, "chain index out of range");
+  return klft_field_download(b->fields[chain], host);
+}
+
+extern "C" int klft_batch_upload(klft_batch_t b, int chain, const double *host) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (chain < 0 || chain >= b->n) return ki::set_error(KLFT_E_ARG, "chain index out of range");
+  return klft_field_upload(b->fields[chain], host);
+}
+
+extern "C" int klft_ptbc_create(int nreplicas, int nc, const int *dims, int storage, double beta, int defect_length,
+                                const int *defect_pos, const double *c, klft_batch_t *out) {
+  if (!out || !dims || !defect_pos || !c || nreplicas < 1) return ki::set_error(KLFT_E_ARG, "invalid PTBC arguments");
+  if (defect_length < 1) return ki::set_error(KLFT_E_ARG, "defect length must be >= 1");
+  for (int k = 1; k < 4; ++k)
+    if (defect_length > dims[k]) return ki::set_error(KLFT_E_ARG, "defect longer than the lattice");
+  std::vector<klft_chain_desc> desc(nreplicas);
+  for (int r = 0; r < nreplicas; ++r) {
+    desc[r].nc = nc;
+    for (int d = 0; d < 4; ++d) desc[r].dims[d] = dims[d];
+    desc[r].beta = beta;
+  }
+  int rc = klft_batch_create(nreplicas, desc.data(), storage, out);
+  if (rc) return rc;
+  klft_batch_t b = *out;
+  b->ptbc = true;
+  b->nc = nc;
+  b->defect.len = defect_length;
+  for (int k = 0; k < 3; ++k) b->defect.pos[k] = ((defect_pos[k] % dims[k + 1]) + dims[k + 1]) % dims[k + 1];
+  b->cvals.assign(c, c + nreplicas);
+  std::vector<int> conf(nreplicas);
+  for (int r = 0; r < nreplicas; ++r) conf[r] = r;
+  hipError_t e = hipMalloc(&b->d_cvals, sizeof(double) * nreplicas);
+  if (e == hipSuccess) e = hipMalloc(&b->d_cval, sizeof(double) * nreplicas);
+  if (e == hipSuccess) e = hipMalloc(&b->d_A, sizeof(double) * 2 * nreplicas);
+  if (e == hipSuccess) e = hipMalloc(&b->d_conf_of, sizeof(int) * nreplicas);
+  if (e == hipSuccess) e = hipMalloc(&b->d_swaps, sizeof(int) * nreplicas);
+  if (e == hipSuccess) e = hipMemcpy(b->d_cvals, c, sizeof(double) * nreplicas, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_cval, c, sizeof(double) * nreplicas, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_conf_of, conf.data(), sizeof(int) * nreplicas, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(b->d_swaps, 0, sizeof(int) * nreplicas);
+  if (e != hipSuccess) {
+    klft_batch_destroy(b);
+    *out = nullptr;
+    return ki::hip_fail(e, "PTBC workspace");
+  }
+  return KLFT_OK;
+}
+
+extern "C" int klft_ptbc_step(klft_batch_t b, int nhits, double delta, uint64_t seed, uint64_t step, int *swaps) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!b->ptbc) return ki::set_error(KLFT_E_STATE, "not a PTBC replica set");
+  rc = ensure_graph(b, nhits, delta, seed);
+  if (rc) return rc;
+  const uint32_t s = (uint32_t)step;
+  KLFT_HIP(hipMemcpyAsync(b->d_sweep, &s, sizeof s, hipMemcpyHostToDevice, ki::stream()));
+  KLFT_HIP(hipGraphLaunch(b->exec, ki::stream()));
+  if (swaps && b->n > 1)
+    KLFT_HIP(hipMemcpyAsync(swaps, b->d_swaps, sizeof(int) * (b->n - 1), hipMemcpyDeviceToHost, ki::stream()));
+  return ki::sync_stream();
+}
+
+extern "C" int klft_ptbc_labels(klft_batch_t b, int *labels) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!b->ptbc || !labels) return ki::set_error(KLFT_E_ARG, "not a PTBC replica set / null output");
+  KLFT_HIP(hipMemcpyAsync(labels, b->d_conf_of, sizeof(int) * b->n, hipMemcpyDeviceToHost, ki::stream()));
+  return ki::sync_stream();
+}
+
+extern "C" int klft_ptbc_action(klft_batch_t b, int replica, int coupling, double *out) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!b->ptbc || !out || replica < 0 || replica >= b->n || coupling < 0 || coupling >= b->n)
+    return ki::set_error(KLFT_E_ARG, "invalid PTBC action request");
+  // S_c(U) = -(beta/Nc) [P_raw(U) - A1 - A2 + c A1 + c^2 A2]
+  double raw = 0.0;
+  rc = klft_plaquette(b->fields[replica], 0, &raw, nullptr);
+  if (rc) return rc;
+  auto launch = [&](auto rt, auto ct) -> int {
+    using R = typename decltype(rt)::type;
+    constexpr bool COMP = decltype(ct)::value != 0;
+    switch (b->nc) {
+      case 1: hipLaunchKernelGGL((k_defect_sums<R, 1, false>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
+      case 2: hipLaunchKernelGGL((k_defect_sums<R, 2, COMP>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
+      default: hipLaunchKernelGGL((k_defect_sums<R, 3, COMP>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
+    }
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  };
+  switch (b->storage) {
+    case KLFT_STORAGE_F64: rc = launch(ki::tag<double>{}, ki::ic<0>{}); break;
+    case KLFT_STORAGE_F32: rc = launch(ki::tag<float>{}, ki::ic<0>{}); break;
+    default: rc = launch(ki::tag<float>{}, ki::ic<1>{}); break;
+  }
+  if (rc) return rc;
+  double A[2];
+  KLFT_HIP(hipMemcpyAsync(A, b->d_A + 2 * replica, sizeof A, hipMemcpyDeviceToHost, ki::stream()));
+  KLFT_HIP(hipStreamSynchronize(ki::stream()));
+  const double c = b->cvals[coupling];
+  *out = -(b->beta[0] / (double)b->nc) * (raw - A[0] - A[1] + c * A[0] + c * c * A[1]);
+  return KLFT_OK;
+}

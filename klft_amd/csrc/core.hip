@@ -476,24 +476,6 @@ extern "C" int klft_field_reunitarize(klft_field_t f) {
 // ============================================================================
 // plaquette (GaugePlaquette.hpp:36-135)
 // ============================================================================
-// p(x) = sum_{mu<nu} Re Tr[(U_mu(x) U_nu(x+mu)) (U_nu(x) U_mu(x+nu))^+]
-template <typename R, int NC, bool COMP, int ND>
-__device__ __forceinline__ double plaq_at(const Links<R, NC, COMP> &f, const Geom &g, const int *x,
-                                          long long s) {
-  double p = 0.0;
-#pragma unroll 1
-  for (int mu = 0; mu < ND - 1; ++mu) {
-    const long long sm = idx_shift<ND>(g, x, mu, 1, -1, 0);
-#pragma unroll 1
-    for (int nu = mu + 1; nu < ND; ++nu) {
-      const Mat<R, NC> lmu = mul(f.load(s, mu), f.load(sm, nu));
-      const Mat<R, NC> lnu = mul(f.load(s, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
-      p += (double)retr_mul_ad(lmu, lnu);
-    }
-  }
-  return p;
-}
-
 template <typename R, int NC, bool COMP, int ND, int OCC>
 __global__ __launch_bounds__(256, OCC) void k_plaquette(Links<R, NC, COMP> f, Geom g, double *partials,
                                                     unsigned int *ticket, double *out,

@@ -520,6 +520,25 @@ __device__ __forceinline__ Mat<R, NC> staple_lowreg(const Links<R, NC, COMP>& f,
   return t;
 }
 
+// plaquette at one site (GaugePlaquette.hpp:52-85), real part only:
+// p(x) = sum_{mu<nu} Re Tr[(U_mu(x) U_nu(x+mu)) (U_nu(x) U_mu(x+nu))^+]
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ double plaq_at(const Links<R, NC, COMP>& f, const Geom& g, const int* x,
+                                          long long s) {
+  double p = 0.0;
+#pragma unroll 1
+  for (int mu = 0; mu < ND - 1; ++mu) {
+    const long long sm = idx_shift<ND>(g, x, mu, 1, -1, 0);
+#pragma unroll 1
+    for (int nu = mu + 1; nu < ND; ++nu) {
+      const Mat<R, NC> lmu = mul(f.load(s, mu), f.load(sm, nu));
+      const Mat<R, NC> lnu = mul(f.load(s, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
+      p += (double)retr_mul_ad(lmu, lnu);
+    }
+  }
+  return p;
+}
+
 // waves per SIMD requested from the register allocator
 template <typename R, int NC>
 struct Occ {

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "metro.hpp"
 
 using namespace kd;
 namespace ki = klft_internal;
@@ -30,38 +31,9 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pass(
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   const long long t = (long long)lb * kMetroBlock + threadIdx.x;
   unsigned long long n = 0;
-  if (t < g.Vh) {
-    int x[ND];
-    sub_coords<ND>(g, pass, t, x);  // index_odd_even
-    const long long s = (long long)pass * g.Vh + t;
-    const uint32_t site = (uint32_t)lex_index<ND>(g, x);
-    if (visit) visit[site] = pass;
-#pragma unroll 1
-    for (int mu = 0; mu < ND; ++mu) {
-      const Mat<R, NC> S = staple<R, NC, COMP, ND>(f, g, x, mu);
-      Mat<R, NC> U = f.load(s, mu);
-      Mat<R, NC> M = mul(S, U);
-      R trM = retrace(M);
-#pragma unroll 1
-      for (int hit = 0; hit < nhits; ++hit) {
-        klft_rng::Stream st(seed, klft_rng::TAG_METROPOLIS, site, sweep, (uint32_t)hit, chain,
-                            (uint32_t)mu);
-        Mat<R, NC> r;
-        rand_sun(st, delta, r);
-        const R dS = -beta_nc * (retr_mul(r, M) - trM);
-        bool accept = dS < R(0);
-        if (!accept) accept = unif<R>(st) < exp(-dS);
-        if (accept) {
-          U = mul(U, r);
-          reunitarize(U);
-          M = mul(S, U);
-          trM = retrace(M);
-          ++n;
-        }
-      }
-      if (nhits > 0) f.store(s, mu, U);
-    }
-  }
+  if (t < g.Vh)
+    n = metro_site<R, NC, COMP, ND, false>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain, Defect{},
+                                           R(1), visit);
   n = wave_sum_u64(n);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
   __syncthreads();
@@ -258,8 +230,11 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
 // scratch copy of the links, so the live field is never updated by it
 int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   void *scratch = nullptr;
-  // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4 waves/SIMD
-  const int v = pick_variant("metropolis", f, 4, 1, [&](int var) -> int {
+  // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4
+  // waves/SIMD.  Untuned default (measured at 32^4): the quad split pays off
+  // only for FP32 SU(3), where the proposal generation dominates a hit.
+  const int dflt = (f->storage != KLFT_STORAGE_F64 && f->nc == 3) ? 2 : 0;
+  const int v = pick_variant("metropolis", f, 4, dflt, [&](int var) -> int {
     if (!scratch) {
       KLFT_HIP(hipMalloc(&scratch, f->link_bytes));
       KLFT_HIP(hipMemcpyAsync(scratch, f->links, f->link_bytes, hipMemcpyDeviceToDevice, stream()));

@@ -1,0 +1,127 @@
+"""GPU parity of the batched-chain and PTBC replica engine against the oracle.
+
+Chains are keyed by their index in the Philox counter, so chain i of a batch
+reproduces the oracle sweep with chain = i (orc.metropolis_sweep_defect with
+all coupling factors 1).  FP64 storage makes the comparison exact up to
+rounding; FP32 batches are checked statistically.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+klft = pytest.importorskip("klft_amd")
+
+CHAINS = [(1, (4, 6, 4, 4), 1.5), (2, (6, 4, 4, 8), 2.1), (3, (4, 4, 6, 4), 5.2), (2, (4, 4, 4, 4), 2.5),
+          (1, (8, 4, 4, 6), 2.0)]
+
+
+def start(nc, dims, i):
+    return orc.random_algebra(4, nc, dims, 0.4, 100 + i)
+
+
+def test_batch_fp64_matches_oracle_per_chain():
+    b = klft.Batch(CHAINS, "f64")
+    for i, (nc, dims, _) in enumerate(CHAINS):
+        b.upload(i, start(nc, dims, i))
+    refs = [start(nc, dims, i) for i, (nc, dims, _) in enumerate(CHAINS)]
+    for sweep in range(2):
+        acc = b.sweep(6, 0.3, 777, sweep)
+        for i, (nc, dims, beta) in enumerate(CHAINS):
+            ones = np.ones((*dims, 4))
+            a, n = orc.metropolis_sweep_defect(refs[i], ones, 4, nc, dims, 6, beta, 0.3, 777, sweep, i)
+            assert acc[i] == a, (sweep, i, acc[i], a)
+    for i, (nc, dims, _) in enumerate(CHAINS):
+        np.testing.assert_allclose(b.download(i), refs[i], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(b.plaquette(), [orc.plaquette(refs[i], 4, nc, dims)
+                                               for i, (nc, dims, _) in enumerate(CHAINS)], atol=1e-13)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_batch_graph_replay_equals_direct_sweeps(storage):
+    b1 = klft.Batch(CHAINS, storage)
+    b2 = klft.Batch(CHAINS, storage)
+    for i, (nc, dims, _) in enumerate(CHAINS):
+        U = start(nc, dims, i)
+        b1.upload(i, U)
+        b2.upload(i, U)
+    for s in range(5):
+        b1.sweep(4, 0.2, 31, 10 + s)
+    b2.run(4, 0.2, 31, 10, 5)
+    for i in range(len(CHAINS)):
+        assert np.array_equal(b1.download(i), b2.download(i))
+
+
+def test_batch_fp32_plaquette_tracks_fp64():
+    """FP32 links with FP64 plaquette sums: the same chains stay within
+    statistical agreement of the FP64 ones after thermalisation."""
+    chains = [(1, (8, 8, 8, 8), 1.0 + 0.1 * k) for k in range(4)] + [(2, (8, 8, 8, 8), 2.0 + 0.1 * k) for k in range(4)]
+    b32, b64 = klft.Batch(chains, "f32"), klft.Batch(chains, "f64")
+    b32.set_identity()
+    b64.set_identity()
+    b32.run(10, 0.5, 5, 0, 150)
+    b64.run(10, 0.5, 5, 0, 150)
+    p32, p64 = [], []
+    for s in range(40):
+        b32.run(10, 0.5, 5, 150 + 5 * s, 5)
+        b64.run(10, 0.5, 5, 150 + 5 * s, 5)
+        p32.append(b32.plaquette())
+        p64.append(b64.plaquette())
+    p32, p64 = np.array(p32), np.array(p64)
+    err = np.sqrt(p32.var(axis=0) / 8 + p64.var(axis=0) / 8) + 1e-4
+    assert np.all(np.abs(p32.mean(axis=0) - p64.mean(axis=0)) < 5 * err)
+
+
+def test_ptbc_fp64_step_matches_oracle():
+    nrep, nc, dims, beta = 4, 3, (4, 4, 4, 4), 5.5
+    L, pos = 2, [1, 0, 2]
+    c = np.linspace(1.0, 0.0, nrep)
+    seed = 4321
+    p = klft.PTBC(nrep, nc, dims, beta, L, pos, c, "f64")
+    refs = [orc.random_algebra(4, nc, dims, 0.3, 50 + r) for r in range(nrep)]
+    for r in range(nrep):
+        p.upload(r, refs[r])
+    conf_of = list(range(nrep))   # conf_of[k]: configuration holding coupling k
+    for step in range(3):
+        swaps = p.step(5, 0.2, seed, step)
+        cval = {conf_of[k]: c[k] for k in range(nrep)}
+        for r in range(nrep):
+            C = orc.defect_field(4, dims, L, pos, cval[r])
+            orc.metropolis_sweep_defect(refs[r], C, 4, nc, dims, 5, beta, 0.2, seed, step, r)
+        for r in range(nrep):
+            np.testing.assert_allclose(p.download(r), refs[r], rtol=0, atol=1e-11)
+
+        def S(U, ck):
+            return -(beta / nc) * orc.plaquette_defect(U, orc.defect_field(4, dims, L, pos, ck), 4, nc, dims,
+                                                       normalize=False)
+        want = []
+        for k in range(nrep - 1):
+            a, b_ = conf_of[k], conf_of[k + 1]
+            dS = S(refs[a], c[k + 1]) + S(refs[b_], c[k]) - S(refs[a], c[k]) - S(refs[b_], c[k + 1])
+            u = orc.stream_doubles(seed, 8, k, step, 0, 0, 0, 1)[0]
+            acc = dS <= 0 or u < np.exp(-dS)
+            want.append(int(acc))
+            if acc:
+                conf_of[k], conf_of[k + 1] = b_, a
+        assert list(swaps) == want
+        assert list(p.labels()) == conf_of
+    for r in range(nrep):
+        for k in range(nrep):
+            want = -(beta / nc) * orc.plaquette_defect(refs[r], orc.defect_field(4, dims, L, pos, c[k]), 4, nc,
+                                                       dims, normalize=False)
+            assert abs(p.action(r, k) - want) < 1e-9 * abs(want)
+
+
+def test_ptbc_c5_shape_runs_in_fp32():
+    """The 16-replica SU(3) 16^4 set: one graph-captured step (sweep of every
+    replica, FP64 swap pass) completes and keeps the labels a permutation."""
+    nrep = 16
+    c = np.linspace(1.0, 0.0, nrep)
+    p = klft.PTBC(nrep, 3, (16, 16, 16, 16), 6.0, 2, [0, 0, 0], c, "f32")
+    p.set_identity()
+    for step in range(3):
+        p.step(8, 0.2, 9, step)
+    assert sorted(p.labels()) == list(range(nrep))
+    pl = p.plaquette()
+    assert np.all((pl > 0.3) & (pl <= 1.0))

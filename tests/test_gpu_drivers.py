@@ -1,0 +1,167 @@
+"""The drop-in drivers on the GPU: `metropolis -f input.yaml` writes the
+reference's three observable files (GaugeObservable.hpp:130-230), whose values
+are checked against an oracle emulation of run_metropolis
+(Metropolis.hpp:180-223) with the same keyed streams; `hmc` writes its
+per-trajectory table (lib/HMC_SU2.cpp:26-29, 64-67); the benchmark clones run.
+"""
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+klft = pytest.importorskip("klft_amd")
+ROOT = Path(__file__).resolve().parents[1]
+BIN = ROOT / "klft_amd"
+
+INPUT = """MetropolisParams:
+  Ndims: 4
+  Nd: 4
+  Nc: {nc}
+  L0: 8
+  L1: 8
+  L2: 8
+  L3: 8
+  nHits: 10
+  nSweep: {nsweep}
+  seed: 32091
+  beta: {beta}
+  delta: 0.1
+GaugeObservableParams:
+  measurement_interval: 10
+  measure_plaquette: true
+  measure_wilson_loop_temporal: true
+  measure_wilson_loop_mu_nu: true
+  W_temp_L_T_pairs:
+    - [2, 2]
+    - [3, 4]
+    - [4, 3]
+    - [4, 4]
+  W_mu_nu_pairs:
+    - [0, 1]
+    - [1, 2]
+    - [3, 2]
+  W_Lmu_Lnu_pairs:
+    - [2, 2]
+    - [3, 3]
+    - [4, 3]
+  plaquette_filename: "plaquette.out"
+  W_temp_filename: "W_temp.out"
+  W_mu_nu_filename: "W_mu_nu.out"
+  write_to_file: true
+"""
+
+
+def emulate(nc, beta, nsweep):
+    """run_metropolis with the oracle; returns the expected file rows."""
+    dims = (8, 8, 8, 8)
+    U = orc.identity(4, nc, dims)
+    plaq, wt, wmn = [], [], []
+    LT = [(2, 2), (3, 4), (4, 3), (4, 4)]
+    MN = [(0, 1), (1, 2), (3, 2)]
+    LL = [(2, 2), (3, 3), (4, 3)]
+    for step in range(nsweep):
+        orc.metropolis_sweep(U, 4, nc, dims, 10, beta, 0.1, 32091, step)
+        if step % 10 == 0 and step != 0:
+            plaq.append((step, orc.plaquette(U, 4, nc, dims)))
+            for (L, T), w in zip(LT, orc.wilson_temporal(U, 4, nc, dims, LT)):
+                wt.append((step, L, T, w))
+            for mu, nu in MN:
+                for (a, b), w in zip(LL, orc.wilson_mu_nu(U, 4, nc, dims, mu, nu, LL)):
+                    wmn.append((step, mu, nu, a, b, w))
+    return plaq, wt, wmn
+
+
+def rows(path):
+    lines = Path(path).read_text().splitlines()
+    return lines[0], [[float(v) for v in l.split(", ")] for l in lines[1:]]
+
+
+@pytest.mark.parametrize("nc,beta", [(1, 2.0), (3, 5.7)])
+def test_metropolis_driver_files_match_oracle(tmp_path, nc, beta):
+    nsweep = 31
+    (tmp_path / "input.yaml").write_text(INPUT.format(nc=nc, nsweep=nsweep, beta=beta))
+    r = subprocess.run([str(BIN / "metropolis"), "-f", "input.yaml"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    plaq, wt, wmn = emulate(nc, beta, nsweep)
+    head, got = rows(tmp_path / "plaquette.out")
+    assert head == "# step, plaquette"
+    assert [g[0] for g in got] == [p[0] for p in plaq]
+    np.testing.assert_allclose([g[1] for g in got], [p[1] for p in plaq], rtol=2e-6)
+    head, got = rows(tmp_path / "W_temp.out")
+    assert head == "# step, L, T, W_temp"
+    np.testing.assert_allclose(np.array(got)[:, :3], np.array(wt)[:, :3])
+    np.testing.assert_allclose(np.array(got)[:, 3], np.array(wt)[:, 3], rtol=2e-5, atol=2e-6)
+    head, got = rows(tmp_path / "W_mu_nu.out")
+    assert head == "# step, mu, nu, Lmu, Lnu, W_mu_nu"
+    np.testing.assert_allclose(np.array(got)[:, :5], np.array(wmn)[:, :5])
+    np.testing.assert_allclose(np.array(got)[:, 5], np.array(wmn)[:, 5], rtol=2e-5, atol=2e-6)
+    # values are printed with the default ostream precision (6 significant digits)
+    line = (tmp_path / "plaquette.out").read_text().splitlines()[1]
+    assert line == f"10, {plaq[0][1]:g}" or abs(float(line.split(", ")[1]) - plaq[0][1]) < 1e-6
+    # a second run appends, header included (Q8)
+    subprocess.run([str(BIN / "metropolis"), "-f", "input.yaml"], cwd=tmp_path, capture_output=True, timeout=120)
+    assert (tmp_path / "plaquette.out").read_text().count("# step, plaquette") == 2
+
+
+def test_write_to_file_disabled(tmp_path):
+    (tmp_path / "input.yaml").write_text(
+        INPUT.format(nc=1, nsweep=12, beta=2.0).replace("write_to_file: true", "write_to_file: false"))
+    r = subprocess.run([str(BIN / "metropolis"), "-f", "input.yaml"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0 and "write_to_file is not enabled" in r.stdout
+    assert not (tmp_path / "plaquette.out").exists()
+
+
+def test_odd_extent_is_rejected(tmp_path):
+    (tmp_path / "input.yaml").write_text(INPUT.format(nc=1, nsweep=2, beta=2.0).replace("L1: 8", "L1: 7"))
+    r = subprocess.run([str(BIN / "metropolis"), "-f", "input.yaml"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 255 and "even" in r.stdout
+
+
+def test_tuning_cache_written_in_reference_shape(tmp_path):
+    (tmp_path / "input.yaml").write_text(INPUT.format(nc=2, nsweep=11, beta=2.2))
+    env = dict(os.environ, KLFT_TUNING="1", KLFT_CACHE_FILE=str(tmp_path / "cache.txt"))
+    r = subprocess.run([str(BIN / "metropolis"), "-f", "input.yaml"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout
+    lines = (tmp_path / "cache.txt").read_text().splitlines()
+    keys = [l.split()[1] for l in lines]
+    assert any(k.startswith("metropolis_") for k in keys) and any(k.startswith("plaquette_") for k in keys)
+    for l in lines:
+        f = l.split()
+        assert int(f[0]) == 4 and len(f) == 2 + 4
+    # the cached choice is reused on the next run
+    r = subprocess.run([str(BIN / "metropolis"), "-f", "input.yaml"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120, env=env)
+    assert r.returncode == 0
+
+
+def test_hmc_driver_output(tmp_path):
+    r = subprocess.run([str(BIN / "hmc"), "--gauge-group", "SU2", "--LX", "4", "--LY", "4", "--LZ", "4", "--LT", "4",
+                        "--n-traj", "4", "--n-steps", "10", "--beta", "2.3", "--outfilename", "hmc.out"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    assert "Starting Plaquette:" in r.stdout and "HMC Time:" in r.stdout
+    lines = (tmp_path / "hmc.out").read_text().splitlines()
+    assert lines[0] == "traj, accept, plaquette, time, acceptance rate"
+    assert len(lines) == 5
+    for i, l in enumerate(lines[1:]):
+        f = l.split(", ")
+        assert int(f[0]) == i and f[1] in ("0", "1") and 0 < float(f[2]) < 1
+
+
+@pytest.mark.parametrize("tool", ["plaq_bench", "wloop_temporal_bench", "metropolis_bench"])
+def test_benchmark_clones_run(tool):
+    r = subprocess.run([str(BIN / tool), "-n", "4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    if tool == "plaq_bench":
+        assert r.stdout.count("Calculated Plaquette value:  1.0000e+00") == 9
+    if tool == "wloop_temporal_bench":
+        assert "Calculated W_temp:  1.0000e+00" in r.stdout

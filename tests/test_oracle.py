@@ -1,0 +1,385 @@
+"""Pins the FP64 CPU oracle (the checker) without a GPU.
+
+Each check is chosen so that a plausible mistake (dropped term, wrong sign or
+index, transposed operand) fails it: published known answers, closed forms on
+special fields, exact invariants, independent numpy / scipy implementations
+and exactly solvable ensembles.  The reference ships no golden vectors of its
+own beyond the cold-field expectation 1.0 of its benchmarks
+(plaq_bench.cpp:88-89, wloop_temporal_bench.cpp:96-100).
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+from scipy.special import iv
+
+import oracle as orc
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+# --------------------------------------------------------------------------
+# random numbers
+# --------------------------------------------------------------------------
+# Random123 known-answer vectors for philox4x32_10 (Salmon et al. SC'11)
+R123_KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+def rocrand_kat():
+    out = []
+    for line in (GOLDEN / "philox_kat.txt").read_text().split("\n"):
+        if not line.strip():
+            continue
+        w = [int(x, 16) for x in line.split()]
+        out.append((w[0:4], w[4:6], w[6:10]))
+    return out
+
+
+@pytest.mark.parametrize("ctr,key,want", R123_KAT + rocrand_kat())
+def test_philox_known_answers(ctr, key, want):
+    assert orc.philox(ctr, key) == want
+
+
+def test_stream_counter_layout():
+    """Block j of stream (tag, c0, c1, c2, chain, mu) uses counter
+    {c0, c1, c2, tag<<28 | chain<<18 | mu<<16 | j}; doubles take 53 bits of
+    two words."""
+    seed, tag, c0, c1, c2, chain, mu = 0x123456789, 1, 77, 5, 3, 2, 1
+    d = orc.stream_doubles(seed, tag, c0, c1, c2, chain, mu, 6)
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    words = []
+    for j in range(3):
+        words += orc.philox([c0, c1, c2, (tag << 28) | (chain << 18) | (mu << 16) | j], key)
+    for i in range(6):
+        a, b = words[2 * i] >> 5, words[2 * i + 1] >> 6
+        assert d[i] == (a * 67108864.0 + b) / 9007199254740992.0
+    assert np.all((d >= 0) & (d < 1))
+
+
+# --------------------------------------------------------------------------
+# SU(N) algebra
+# --------------------------------------------------------------------------
+def rand_c(n, rng):
+    return rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_matmul_matches_numpy(nc):
+    rng = np.random.default_rng(nc)
+    a, b = rand_c(nc, rng), rand_c(nc, rng)
+    np.testing.assert_allclose(orc.matmul(a, b), a @ b, rtol=0, atol=1e-13)
+
+
+def haar(nc, rng):
+    z = rand_c(nc, rng)
+    q, r = np.linalg.qr(z)
+    q = q * (np.diag(r) / np.abs(np.diag(r)))
+    return q / np.linalg.det(q) ** (1.0 / nc)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_restore_sun_fixed_point_and_projection(nc):
+    rng = np.random.default_rng(10 + nc)
+    u = haar(nc, rng)
+    np.testing.assert_allclose(orc.restore_sun(u), u, atol=1e-14)
+    w = orc.restore_sun(u + 1e-3 * rand_c(nc, rng))
+    # row 0 normalised (all rows for SU(2)); the completed rows are cross products
+    assert abs(np.linalg.norm(w[0]) - 1) < 1e-14
+    if nc == 3:
+        assert abs(np.vdot(w[0], w[2])) < 1e-13 and abs(np.vdot(w[1], w[2])) < 1e-6
+        np.testing.assert_allclose(w[2], np.conj(np.cross(w[0], w[1])), atol=1e-14)
+        np.testing.assert_allclose(w[1], np.conj(np.cross(w[2], w[0])), atol=1e-14)
+    if nc <= 2:
+        # U(1)/SU(2): one real positive rescaling of the whole matrix (Q2: row 1
+        # is not re-orthogonalised)
+        u2 = u + 1e-3 * rand_c(nc, np.random.default_rng(11 + nc))
+        w2 = orc.restore_sun(u2)
+        s = (u2 / w2).ravel()
+        assert np.allclose(s.imag, 0, atol=1e-12) and np.allclose(s.real, s.real[0]) and s.real[0] > 0
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_rand_sun_is_special_unitary(nc):
+    for hit in range(20):
+        r, _ = orc.rand_sun(nc, 0.3, 99, 0, 17, 1, hit)
+        np.testing.assert_allclose(r @ r.conj().T, np.eye(nc), atol=1e-14)
+        if nc > 1:
+            assert abs(np.linalg.det(r) - 1) < 1e-14
+
+
+def test_rand_sun_laws():
+    # U(1): phase uniform in [-delta pi, delta pi)
+    th = [np.angle(orc.rand_sun(1, 0.2, 5, 0, s, 0, 0)[0][0, 0]) for s in range(400)]
+    assert max(np.abs(th)) <= 0.2 * np.pi + 1e-12 and np.std(th) > 0.3
+    # SU(2): rotation angle alpha in [0, 2 pi delta): Re r00 = cos(alpha)
+    c = [orc.rand_sun(2, 0.1, 5, 0, s, 0, 0)[0][0, 0].real for s in range(400)]
+    assert min(c) >= np.cos(2 * np.pi * 0.1) - 1e-12
+    # SU(3): first row components in the first quadrant, independent of delta
+    r1, _ = orc.rand_sun(3, 0.1, 5, 0, 3, 0, 0)
+    r2, _ = orc.rand_sun(3, 0.9, 5, 0, 3, 0, 0)
+    assert np.all(r1[0].real >= 0) and np.all(r1[0].imag >= 0)
+    np.testing.assert_allclose(r1, r2, atol=1e-13)
+
+
+def su_generators(nc):
+    if nc == 1:
+        return np.ones((1, 1, 1), complex)
+    if nc == 2:
+        return np.array([[[0, 1], [1, 0]], [[0, -1j], [1j, 0]], [[1, 0], [0, -1]]], complex)
+    lam = np.zeros((8, 3, 3), complex)
+    lam[0][0, 1] = lam[0][1, 0] = 1
+    lam[1][0, 1], lam[1][1, 0] = -1j, 1j
+    lam[2][0, 0], lam[2][1, 1] = 1, -1
+    lam[3][0, 2] = lam[3][2, 0] = 1
+    lam[4][0, 2], lam[4][2, 0] = -1j, 1j
+    lam[5][1, 2] = lam[5][2, 1] = 1
+    lam[6][1, 2], lam[6][2, 1] = -1j, 1j
+    lam[7] = np.diag([1, 1, -2]) / np.sqrt(3)
+    return lam
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("scale", [0.0, 1e-6, 0.05, 0.7, 3.0])
+def test_exp_matches_expm(nc, scale):
+    rng = np.random.default_rng(int(scale * 100) + nc)
+    T = su_generators(nc)
+    v = scale * rng.normal(size=len(T))
+    want = expm(1j * np.einsum("a,aij->ij", v, T))
+    np.testing.assert_allclose(orc.exp_iq(nc, v), want, rtol=0, atol=1e-14 * max(1, scale))
+
+
+# --------------------------------------------------------------------------
+# fields, plaquette, staple, Wilson loops
+# --------------------------------------------------------------------------
+def random_field(nd, nc, dims, seed):
+    return orc.random_algebra(nd, nc, dims, 0.8, seed)
+
+
+def gauge_transform(U, nd, nc, dims, seed):
+    rng = np.random.default_rng(seed)
+    g = np.array([haar(nc, rng) for _ in range(int(np.prod(dims)))]).reshape(*dims, nc, nc)
+    V = U.copy()
+    for mu in range(nd):
+        gp = np.roll(g, -1, axis=mu)
+        V[..., mu, :, :] = g @ U[..., mu, :, :] @ np.conj(np.swapaxes(gp, -1, -2))
+    return np.ascontiguousarray(V), g
+
+
+def plane_plaq_numpy(U, mu, nu):
+    """Independent vectorised plaquette of one plane, per site."""
+    Um, Un = U[..., mu, :, :], U[..., nu, :, :]
+    p = Um @ np.roll(Un, -1, axis=mu) @ np.conj(np.swapaxes(np.roll(Um, -1, axis=nu), -1, -2)) @ \
+        np.conj(np.swapaxes(Un, -1, -2))
+    return np.trace(p, axis1=-2, axis2=-1).real
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", [(2, (4, 6)), (3, (4, 4, 6)), (4, (4, 4, 4, 6))])
+def test_plaquette_cold_pure_gauge_and_numpy(nd, nc, dims):
+    assert orc.plaquette(orc.identity(nd, nc, dims), nd, nc, dims) == 1.0
+    V, _ = gauge_transform(orc.identity(nd, nc, dims), nd, nc, dims, 3)
+    assert abs(orc.plaquette(V, nd, nc, dims) - 1.0) < 1e-14
+    U = random_field(nd, nc, dims, 4)
+    ref = sum(plane_plaq_numpy(U, m, n).sum() for m in range(nd) for n in range(m + 1, nd))
+    ref /= np.prod(dims) * nd * (nd - 1) // 2 * nc
+    assert abs(orc.plaquette(U, nd, nc, dims) - ref) < 1e-13
+    # gauge invariance
+    W, _ = gauge_transform(U, nd, nc, dims, 5)
+    assert abs(orc.plaquette(W, nd, nc, dims) - orc.plaquette(U, nd, nc, dims)) < 1e-13
+
+
+def test_u1_constant_flux():
+    nd, dims = 4, (6, 8, 4, 4)
+    k = 3
+    th = 2 * np.pi * k / dims[1]
+    U = orc.identity(nd, 1, dims)
+    x1 = np.arange(dims[1]).reshape(1, -1, 1, 1)
+    U[..., 0, 0, 0] = np.exp(1j * th * x1)
+    # only the (0,1) plane carries flux: P = (5 + cos th) / 6
+    assert abs(orc.plaquette(U, nd, 1, dims) - (5 + np.cos(th)) / 6) < 1e-14
+    w = orc.wilson_mu_nu(U, nd, 1, dims, 0, 1, [(1, 1), (2, 3), (3, 2)])
+    np.testing.assert_allclose(w, [np.cos(th * 1), np.cos(th * 6), np.cos(th * 6)], atol=1e-13)
+
+
+def test_temporal_loop_flux():
+    nd, dims = 4, (4, 4, 4, 8)
+    th = 2 * np.pi / dims[0]
+    U = orc.identity(nd, 1, dims)
+    x0 = np.arange(dims[0]).reshape(-1, 1, 1, 1)
+    U[..., 3, 0, 0] = np.exp(1j * th * x0 * np.ones(dims))  # flux in plane (0, 3) only
+    pairs = [(1, 1), (2, 3), (3, 4)]
+    got = orc.wilson_temporal(U, nd, 1, dims, pairs)
+    want = [(np.cos(th * L * T) + 2) / 3 for L, T in pairs]
+    np.testing.assert_allclose(got, want, atol=1e-13)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_wilson_loops_against_numpy_lines(nc):
+    nd, dims = 4, (4, 6, 4, 6)
+    U = random_field(nd, nc, dims, 8)
+    assert np.allclose(orc.wilson_mu_nu(orc.identity(nd, nc, dims), nd, nc, dims, 1, 3, [(2, 2)]), 1.0)
+
+    def line(mu, L):
+        out = np.broadcast_to(np.eye(nc, dtype=complex), (*dims, nc, nc)).copy()
+        for i in range(L):
+            out = out @ np.roll(U[..., mu, :, :], -i, axis=mu)
+        return out
+
+    for mu, nu in [(0, 1), (2, 3), (3, 1)]:
+        pairs = [(1, 1), (2, 3), (3, 2), (1, 4)]
+        got = orc.wilson_mu_nu(U, nd, nc, dims, mu, nu, pairs)
+        for (Lm, Ln), g in zip(pairs, got):
+            a = line(mu, Lm) @ np.roll(line(nu, Ln), -Lm, axis=mu)
+            b = line(nu, Ln) @ np.roll(line(mu, Lm), -Ln, axis=nu)
+            w = np.einsum("...ij,...ij->...", a, np.conj(b)).real.sum() / np.prod(dims) / nc
+            assert abs(g - w) < 1e-13
+        # W(1,1) of a plane is that plane's plaquette
+        assert abs(orc.wilson_mu_nu(U, nd, nc, dims, mu, nu, [(1, 1)])[0] -
+                   plane_plaq_numpy(U, mu, nu).mean() / nc) < 1e-13
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", [(2, (4, 6)), (4, (4, 4, 4, 6))])
+def test_staple_identities(nd, nc, dims):
+    S = orc.staple_field(orc.identity(nd, nc, dims), nd, nc, dims)
+    np.testing.assert_allclose(S, 2 * (nd - 1) * np.broadcast_to(np.eye(nc), S.shape), atol=1e-15)
+    U = random_field(nd, nc, dims, 6)
+    S = orc.staple_field(U, nd, nc, dims)
+    tot = np.einsum("...ij,...ji->...", U, S).real.sum()
+    assert abs(tot - 4 * orc.plaquette(U, nd, nc, dims, normalize=False)) < 1e-10
+    # gauge covariance: S_mu(x) -> g(x+mu) S_mu(x) g(x)^+
+    W, g = gauge_transform(U, nd, nc, dims, 7)
+    SW = orc.staple_field(W, nd, nc, dims)
+    for mu in range(nd):
+        want = np.roll(g, -1, axis=mu) @ S[..., mu, :, :] @ np.conj(np.swapaxes(g, -1, -2))
+        np.testing.assert_allclose(SW[..., mu, :, :], want, atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# Metropolis
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nd,dims", [(2, (4, 6)), (3, (4, 2, 6)), (4, (4, 4, 2, 6))])
+def test_schedule_is_parity_class(nd, dims):
+    U = orc.identity(nd, 1, dims)
+    _, _, visit = orc.metropolis_sweep(U, nd, 1, dims, 0, 1.0, 0.1, 1, 0, visit=True)
+    x = np.indices(dims)
+    want = sum((x[d] & 1) << (nd - 1 - d) for d in range(nd)).reshape(-1)
+    assert np.array_equal(visit, want)
+
+
+def test_metropolis_acceptance_normalisation_and_determinism():
+    nd, nc, dims = 4, 2, (4, 4, 4, 4)
+    U1, U2 = orc.identity(nd, nc, dims), orc.identity(nd, nc, dims)
+    a1, n1 = orc.metropolis_sweep(U1, nd, nc, dims, 5, 2.3, 0.2, 11, 0)
+    a2, n2 = orc.metropolis_sweep(U2, nd, nc, dims, 5, 2.3, 0.2, 11, 0)
+    assert n1 == n2 and np.array_equal(U1, U2)
+    assert a1 == n1 / (np.prod(dims) * nd * 5)
+
+
+def binned_mean(x, nbin=10):
+    b = np.array_split(np.asarray(x), nbin)
+    m = np.array([v.mean() for v in b])
+    return m.mean(), m.std(ddof=1) / np.sqrt(nbin)
+
+
+@pytest.mark.parametrize("nc,beta,exact", [(1, 2.0, lambda b: iv(1, b) / iv(0, b)),
+                                           (2, 2.0, lambda b: iv(2, b) / iv(1, b))])
+def test_metropolis_2d_exact_plaquette(nc, beta, exact):
+    """2D pure gauge theory: <P> = I1(b)/I0(b) for U(1), I2(b)/I1(b) for SU(2)."""
+    nd, dims = 2, (16, 16)
+    U = orc.identity(nd, nc, dims)
+    meas = []
+    for s in range(600):
+        orc.metropolis_sweep(U, nd, nc, dims, 10, beta, 0.5 if nc == 1 else 0.3, 2024, s)
+        if s >= 100:
+            meas.append(orc.plaquette(U, nd, nc, dims))
+    m, e = binned_mean(meas)
+    assert abs(m - exact(beta)) < 4 * e + 2e-3, (m, e, exact(beta))
+
+
+# --------------------------------------------------------------------------
+# HMC
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_force_is_action_derivative(nc):
+    """F_a(x,mu) = d S / d w_a with U_mu(x) -> exp(i w_a T_a) U_mu(x)."""
+    nd, dims, beta = 4, (4, 4, 4, 4), 2.3
+    U = random_field(nd, nc, dims, 12)
+    F = orc.force(U, nd, nc, dims, beta)
+    T = su_generators(nc)
+    eps = 1e-5
+    for site, mu, a in [(0, 0, 0), (17, 2, len(T) - 1), (101, 3, len(T) // 2)]:
+        x = np.unravel_index(site, dims)
+        vals = []
+        for sgn in (1, -1):
+            V = U.copy()
+            V[x + (mu,)] = expm(1j * sgn * eps * T[a]) @ U[x + (mu,)]
+            vals.append(orc.gauge_action(V, nd, nc, dims, beta))
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        assert abs(F[x + (mu, a)] - fd) < 1e-7 * max(1, abs(fd))
+    assert np.abs(orc.force(orc.identity(nd, nc, dims), nd, nc, dims, beta)).max() < 1e-15
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3])
+def test_leapfrog_reversible_and_second_order(nc):
+    nd, dims, beta = 4, (4, 4, 4, 4), 2.0
+    U0 = random_field(nd, nc, dims, 21)
+    P0 = orc.refresh_momenta(nd, nc, dims, 5, 0)
+    U, P = U0.copy(), P0.copy()
+    orc.leapfrog(U, P, nd, nc, dims, beta, 0.5, 10)
+    P *= -1
+    orc.leapfrog(U, P, nd, nc, dims, beta, 0.5, 10)
+    np.testing.assert_allclose(U, U0, atol=1e-11)
+    np.testing.assert_allclose(-P, P0, atol=1e-10)
+
+    def dH(n):
+        U, P = U0.copy(), P0.copy()
+        h0 = orc.kinetic(P, nd, nc, dims) + orc.gauge_action(U, nd, nc, dims, beta)
+        orc.leapfrog(U, P, nd, nc, dims, beta, 0.5, n)
+        return orc.kinetic(P, nd, nc, dims) + orc.gauge_action(U, nd, nc, dims, beta) - h0
+
+    r = abs(dH(8)) / abs(dH(16))
+    assert 3.0 < r < 5.5, r  # dH ~ dtau^2
+
+
+def test_hmc_exp_minus_dh_is_one():
+    nd, nc, dims, beta = 2, 1, (8, 8), 1.5
+    U = orc.identity(nd, nc, dims)
+    vals = []
+    for t in range(300):
+        _, d, _ = orc.hmc_trajectory(U, nd, nc, dims, beta, 1.0, 6, 9, t)
+        if t >= 20:
+            vals.append(np.exp(-d))
+    m, e = binned_mean(vals)
+    assert abs(m - 1.0) < 4 * e + 1e-3, (m, e)
+
+
+def test_defect_field_and_plaquette():
+    nd, nc, dims = 4, 3, (4, 4, 4, 4)
+    C = orc.defect_field(nd, dims, 2, [1, 1, 1], 0.5)
+    assert (C == 0.5).sum() == 8 and (C[..., 1:] == 1).all()
+    assert (C[0, 1:3, 1:3, 1:3, 0] == 0.5).all()
+    U = random_field(nd, nc, dims, 3)
+    assert orc.plaquette_defect(U, np.ones_like(C), nd, nc, dims) == orc.plaquette(U, nd, nc, dims)
+    # cold field: each plaquette through one defect link carries c, two links c^2
+    Ucold = orc.identity(nd, nc, dims)
+    got = orc.plaquette_defect(Ucold, C, nd, nc, dims, normalize=False) / nc
+    nplaq = np.prod(dims) * 6
+    # brute-force count of (0,nu) plaquettes through one or two defect links
+    cnt1 = cnt2 = 0
+    for x in np.ndindex(dims):
+        for nu in range(1, 4):
+            xp = list(x)
+            xp[nu] = (xp[nu] + 1) % dims[nu]
+            n = int(C[x + (0,)] != 1) + int(C[tuple(xp) + (0,)] != 1)
+            cnt1 += n == 1
+            cnt2 += n == 2
+    # 8 defect links in 3 planes x 2 plaquettes = 48 incidences; 12 plaquettes hold two
+    assert (cnt1, cnt2) == (24, 12)
+    want = nplaq - cnt1 - cnt2 + cnt1 * 0.5 + cnt2 * 0.25
+    assert abs(got - want) < 1e-12
