@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--hmc-traj", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="Metropolis lines only")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy the FP64 link field 3x first (known-size kernel that scales PMC byte counters)")
+    ap.add_argument("--only", default=None, help="comma list of precisions to run (f64,f32,f32_rows)")
     return ap.parse_args()
 
 
@@ -148,7 +151,16 @@ def main():
     V = a.L ** 4
     legs, extra = {}, {}
     g64 = None
-    for st in ["f64", "f32", "f32_rows"]:
+    if a.copy_probe:
+        src = klft.GaugeField(4, 3, dims, "f64").set_identity()
+        dst = klft.GaugeField(4, 3, dims, "f64")
+        for _ in range(3):
+            dst.copy_from(src)
+        print(f"[bench] copy probe: 3 x {4 * V * 144} bytes read + written", file=sys.stderr, flush=True)
+        src.close()
+        dst.close()
+    precisions = a.only.split(",") if a.only else ["f64", "f32", "f32_rows"]
+    for st in precisions:
         g, legs[st] = metropolis_leg(klft, st, a, dims)
         print(f"[bench] metropolis {st}: {legs[st]['link_updates_per_s']:.3e} lu/s, "
               f"pass {legs[st]['pass_us']:.1f} us", file=sys.stderr, flush=True)
@@ -159,12 +171,12 @@ def main():
         else:
             g.close()
     if not a.no_extra:
-        for st in ["f64", "f32", "f32_rows"]:
+        for st in precisions:
             extra[st] = hmc_leg(klft, st, a, dims)
             print(f"[bench] hmc {st}: {extra[st]['traj_per_s']:.2f} traj/s", file=sys.stderr, flush=True)
     del g64
 
-    main_leg = legs["f64"]
+    main_leg = legs["f64"] if "f64" in legs else legs[precisions[0]]
     traffic = None
     tj = traffic_from_profiles()
     if tj and "f64" in tj:

@@ -58,19 +58,19 @@ __global__ __launch_bounds__(kB) void k_batch_pass(const ChainDev *__restrict__ 
     const R cc = DEFECT ? (R)cval[c] : R(1);
     switch (ch.nc) {
       case 1: {
-        Links<R, 1, false> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V};
+        Links<R, 1, false> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
         n = metro_site<R, 1, false, 4, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 1.0), delta, seed, sweep,
                                                (uint32_t)c, d, cc, nullptr);
         break;
       }
       case 2: {
-        Links<R, 2, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V};
+        Links<R, 2, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
         n = metro_site<R, 2, COMP, 4, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 2.0), delta, seed, sweep,
                                               (uint32_t)c, d, cc, nullptr);
         break;
       }
       default: {
-        Links<R, 3, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V};
+        Links<R, 3, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
         n = metro_site<R, 3, COMP, 4, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 3.0), delta, seed, sweep,
                                               (uint32_t)c, d, cc, nullptr);
         break;
@@ -100,9 +100,9 @@ __global__ __launch_bounds__(kB) void k_batch_plaq(const ChainDev *__restrict__ 
     int x[4];
     dev_coords<4>(ch.g, s, x);
     switch (ch.nc) {
-      case 1: v = plaq_at<R, 1, false, 4>(Links<R, 1, false>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V}, ch.g, x, s); break;
-      case 2: v = plaq_at<R, 2, COMP, 4>(Links<R, 2, COMP>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V}, ch.g, x, s); break;
-      default: v = plaq_at<R, 3, COMP, 4>(Links<R, 3, COMP>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V}, ch.g, x, s); break;
+      case 1: v = plaq_at<R, 1, false, 4>(Links<R, 1, false>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4}, ch.g, x, s); break;
+      case 2: v = plaq_at<R, 2, COMP, 4>(Links<R, 2, COMP>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4}, ch.g, x, s); break;
+      default: v = plaq_at<R, 3, COMP, 4>(Links<R, 3, COMP>{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4}, ch.g, x, s); break;
     }
   }
   const double bt = block_sum<kB>(v, lds);
@@ -130,7 +130,7 @@ __global__ void k_defect_sums(const ChainDev *__restrict__ chains, int n, Defect
   if (c >= n) return;
   const ChainDev ch = chains[c];
   const Geom &g = ch.g;
-  Links<R, NC, COMP> f{reinterpret_cast<cx<R> *>(ch.links), g.V};
+  Links<R, NC, COMP> f{reinterpret_cast<cx<R> *>(ch.links), g.V, 4};
   double a1 = 0.0, a2 = 0.0;
   for (int nu = 1; nu < 4; ++nu) {
     // base sites y: y0 = 0, y_nu in [p_nu - 1, p_nu + Ld), the others in [p_k, p_k + Ld)

@@ -294,7 +294,7 @@ extern "C" int klft_field_create(int nd, int nc, const int *dims, int storage, k
   const bool comp = storage == KLFT_STORAGE_F32_ROWS && nc > 1;
   f->ncomp = comp ? (nc - 1) * nc : nc * nc;
   f->cbytes = storage == KLFT_STORAGE_F64 ? 16 : 8;
-  f->link_bytes = (size_t)V * nd * f->ncomp * f->cbytes;
+  f->link_bytes = (size_t)kd::links_elems(V, nd, f->ncomp) * f->cbytes;
   hipError_t e = hipMalloc(&f->links, f->link_bytes);
   if (e != hipSuccess) {
     delete f;

@@ -438,18 +438,29 @@ __device__ __forceinline__ long long idx_shift_k(const Geom& g, const int* x, in
 // ---------------------------------------------------------------------------
 // link field access
 // ---------------------------------------------------------------------------
+// Array-of-structs-of-arrays: sites are grouped in blocks of 64 consecutive
+// device indices, and a block stores all directions and colour components of
+// its 64 sites contiguously,
+//   element (s, mu, c) at ((s / 64) * nd + mu) * NCOMP * 64 + c * 64 + s % 64,
+// so a wave's load of one link for 64 sites is one contiguous run of
+// NCOMP * 64 complex numbers (9 KiB for FP64 SU(3)) instead of NCOMP runs in
+// arrays a field-size apart.
 template <typename R, int NC, bool COMP>
 struct Links {
   static constexpr bool C = COMP && NC > 1;
   static constexpr int NCOMP = C ? (NC - 1) * NC : NC * NC;
   cx<R>* __restrict__ p;
   long long V;
+  int nd;
 
+  __device__ __forceinline__ long long base(long long s, int mu) const {
+    return (((s >> 6) * nd + mu) * NCOMP) * 64 + (s & 63);
+  }
   __device__ __forceinline__ Mat<R, NC> load(long long s, int mu) const {
     Mat<R, NC> u;
-    const cx<R>* q = p + (long long)mu * NCOMP * V + s;
+    const cx<R>* q = p + base(s, mu);
 #pragma unroll
-    for (int c = 0; c < NCOMP; ++c) u.m[c / NC][c % NC] = q[(long long)c * V];
+    for (int c = 0; c < NCOMP; ++c) u.m[c / NC][c % NC] = q[c * 64];
     if constexpr (C) {
       if constexpr (NC == 3) complete_row2(u);
       if constexpr (NC == 2) complete_row1(u);
@@ -457,11 +468,16 @@ struct Links {
     return u;
   }
   __device__ __forceinline__ void store(long long s, int mu, const Mat<R, NC>& u) const {
-    cx<R>* q = p + (long long)mu * NCOMP * V + s;
+    cx<R>* q = p + base(s, mu);
 #pragma unroll
-    for (int c = 0; c < NCOMP; ++c) q[(long long)c * V] = u.m[c / NC][c % NC];
+    for (int c = 0; c < NCOMP; ++c) q[c * 64] = u.m[c / NC][c % NC];
   }
 };
+
+// number of link-array elements (complex) for V sites
+__host__ __device__ inline long long links_elems(long long V, int nd, int ncomp) {
+  return ((V + 63) / 64) * 64 * nd * ncomp;
+}
 
 // ---------------------------------------------------------------------------
 // staple, GaugeField.hpp:187-237 (3D :389-433, 2D :579-618):

@@ -138,6 +138,7 @@ kd::Links<R, NC, COMP> links_of(const klft_field_s *f) {
   kd::Links<R, NC, COMP> l;
   l.p = reinterpret_cast<kd::cx<R> *>(f->links);
   l.V = f->g.V;
+  l.nd = f->nd;
   return l;
 }
 

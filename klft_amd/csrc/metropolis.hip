@@ -203,6 +203,7 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
     Links<R, NC, COMP> l;
     l.p = reinterpret_cast<cx<R> *>(links);
     l.V = f->g.V;
+    l.nd = f->nd;
     const R beta_nc = (R)(p->beta / (double)NC);
     const int lanes = variant >= 1 ? 4 : 1;
     const unsigned nb = blocks_for(f->g.Vh * lanes, kMetroBlock);
