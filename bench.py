@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy the FP64 link field 3x first (known-size kernel that scales PMC byte counters)")
     ap.add_argument("--only", default=None, help="comma list of precisions to run (f64,f32,f32_rows)")
+    ap.add_argument("--config", default=None,
+                    help="per-configuration reports instead of the headline line: c1,...,c5 or all")
     return ap.parse_args()
 
 
@@ -139,6 +141,229 @@ def traffic_from_profiles():
     return None
 
 
+# ---------------------------------------------------------------------------
+# per-configuration reports (--config c1..c5): one JSON line each
+# ---------------------------------------------------------------------------
+C1_YAML = """MetropolisParams:
+  Ndims: 4
+  Nd: 4
+  Nc: 1
+  L0: 8
+  L1: 8
+  L2: 8
+  L3: 8
+  nHits: 10
+  nSweep: {nsweep}
+  seed: 32091
+  beta: 2.0
+  delta: 0.1
+GaugeObservableParams:
+  measurement_interval: 10
+  measure_plaquette: true
+  measure_wilson_loop_temporal: true
+  measure_wilson_loop_mu_nu: true
+  W_temp_L_T_pairs: [[2, 2], [3, 4], [4, 3], [4, 4]]
+  W_mu_nu_pairs: [[0, 1], [1, 2], [3, 2]]
+  W_Lmu_Lnu_pairs: [[2, 2], [3, 3], [4, 3]]
+  plaquette_filename: "plaquette.out"
+  W_temp_filename: "W_temp.out"
+  W_mu_nu_filename: "W_mu_nu.out"
+  write_to_file: true
+"""
+
+
+def config_c1(klft, a):
+    """U(1) 8^4, the README input: the metropolis driver end to end (1000
+    sweeps, 99 measurements, three files), timed, beside the oracle running the
+    same keyed chain on the host cores."""
+    import subprocess
+    import tempfile
+    nsweep = 1000
+    with tempfile.TemporaryDirectory() as d:
+        Path(d, "input.yaml").write_text(C1_YAML.format(nsweep=nsweep))
+        t = sync_time(lambda: subprocess.run([str(ROOT / "klft_amd" / "metropolis"), "-f", "input.yaml"], cwd=d,
+                                             check=True, capture_output=True))
+        rows = Path(d, "plaquette.out").read_text().splitlines()[1:]
+    plaq = [float(r.split(", ")[1]) for r in rows]
+    out = {"config": "C1 U(1) 8^4 Metropolis, README input, 1000 sweeps", "gpu_driver_s": t,
+           "sweeps_per_s": nsweep / t, "link_updates_per_s": 4 * 8 ** 4 * nsweep / t,
+           "measurements": len(plaq), "plaquette_last": plaq[-1], "plaquette_mean_2nd_half": float(sum(plaq[49:]) / len(plaq[49:]))}
+    if not a.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle as orc
+        dims = (8, 8, 8, 8)
+        U = orc.identity(4, 1, dims)
+        cpu_plaq = []
+
+        def run():
+            for s in range(nsweep):
+                orc.metropolis_sweep(U, 4, 1, dims, 10, 2.0, 0.1, 32091, s)
+                if s % 10 == 0 and s:
+                    cpu_plaq.append(orc.plaquette(U, 4, 1, dims))
+                    orc.wilson_temporal(U, 4, 1, dims, [(2, 2), (3, 4), (4, 3), (4, 4)])
+                    for mu, nu in [(0, 1), (1, 2), (3, 2)]:
+                        orc.wilson_mu_nu(U, 4, 1, dims, mu, nu, [(2, 2), (3, 3), (4, 3)])
+        tc = sync_time(run)
+        out["cpu_baseline"] = {"value": 4 * 8 ** 4 * nsweep / tc, "unit": "link-updates/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                               "sample": f"the same 1000-sweep keyed chain with measurements, {tc:.1f} s"}
+        out["max_abs_plaquette_diff_vs_cpu"] = max(abs(x - y) for x, y in zip(plaq, cpu_plaq))
+    return out
+
+
+def config_c2(klft, a):
+    """SU(2) 16^4 HMC, beta 2.3, Haar hot start, tau 1, 20 steps, 200
+    trajectories: FP64 links vs FP32 links (+ FP64 dH and action sums)."""
+    import numpy as np
+    res = {"config": "C2 SU(2) 16^4 HMC beta=2.3 tau=1 n=20, 200 trajectories"}
+    for st in ["f64", "f32"]:
+        g = klft.GaugeField(4, 2, (16, 16, 16, 16), st).set_haar(2 * 1234)
+        h = klft.HMC(g, beta=2.3, tau=1.0, nsteps=20, seed=1234)
+        dH, acc, plaq = [], [], []
+        t0 = time.perf_counter()
+        for i in range(200):
+            d, ok = h.trajectory(i)
+            dH.append(d)
+            acc.append(ok)
+            plaq.append(klft.GaugePlaquette(g))
+        t = time.perf_counter() - t0
+        therm = 50
+        e = np.exp(-np.array(dH[therm:]))
+        p = np.array(plaq[therm:])
+        nb = 10
+        eb = np.array([b.mean() for b in np.array_split(e, nb)])
+        pb = np.array([b.mean() for b in np.array_split(p, nb)])
+        res[st] = {"traj_per_s": 200 / t, "acceptance": float(np.mean(acc)),
+                   "plaquette": float(p.mean()), "plaquette_err": float(pb.std(ddof=1) / np.sqrt(nb)),
+                   "exp_minus_dH": float(e.mean()), "exp_minus_dH_err": float(eb.std(ddof=1) / np.sqrt(nb)),
+                   "mean_abs_dH": float(np.abs(dH[therm:]).mean())}
+        h.close()
+        g.close()
+    d = res["f32"]["plaquette"] - res["f64"]["plaquette"]
+    res["plaquette_fp32_minus_fp64_sigma"] = d / np.hypot(res["f32"]["plaquette_err"], res["f64"]["plaquette_err"])
+    return res
+
+
+def config_c3(klft, a):
+    """SU(3) 32^4 fixed fields exp(i eps H), eps in {0.1, 0.5, 1.0}: plaquette,
+    staple field, HMC force and temporal Wilson loops (L, T = 1..8); FP32 and
+    FP32 two-row (re-unitarised) against FP64, with kernel times and GB/s."""
+    import numpy as np
+    dims = (32, 32, 32, 32)
+    V = 32 ** 4
+    pairs = [(L, T) for L in range(1, 9) for T in range(1, 9)]
+    res = {"config": "C3 SU(3) 32^4 measurement + force kernels, eps in {0.1,0.5,1.0}"}
+    for eps in [0.1, 0.5, 1.0]:
+        row = {}
+        ref = {}
+        for st in ["f64", "f32", "f32_rows"]:
+            g = klft.GaugeField(4, 3, dims, st).set_random_algebra(eps, 3000 + int(10 * eps))
+            if st != "f64":
+                g.reunitarize()
+            klft.profile(True)
+            P = [klft.GaugePlaquette(g) for _ in range(5)][-1]
+            pms, pn = klft.profile_read("plaquette")
+            W = np.array([w for _, _, w in klft.WilsonLoop_temporal(g, pairs)])
+            wms, _ = klft.profile_read("wilson")
+            s = klft.stapleField(g)
+            for _ in range(3):
+                klft.lib.klft_staple_field(g.h, s.h)
+            sms, sn = klft.profile_read("staple_field")
+            h = klft.HMC(g, beta=6.0, tau=1.0, nsteps=20, seed=1)
+            F = h.force()
+            for _ in range(3):
+                h.update_momentum(0.0)
+            fms, fn = klft.profile_read("force")
+            klft.profile(False)
+            Sd = s.download()
+            b = BYTES_PER_LINK[st]
+            bs = 2 * 9 * (8 if st == "f64" else 4)
+            entry = {"plaquette": P, "plaquette_us": 1e3 * pms / pn, "plaquette_GBs": 4 * V * b / (1e-3 * pms / pn) / 1e9,
+                     "staple_us": 1e3 * sms / sn, "staple_GBs": 4 * V * (b + bs) / (1e-3 * sms / sn) / 1e9,
+                     "force_us": 1e3 * fms / fn,
+                     "force_GBs": 4 * V * (b + 2 * 8 * (8 if st == 'f64' else 4)) / (1e-3 * fms / fn) / 1e9,
+                     "wilson_64pairs_ms": wms}
+            if st == "f64":
+                ref = {"P": P, "W": W, "S": Sd, "F": F}
+            else:
+                entry["plaquette_err_vs_fp64"] = abs(P - ref["P"])
+                entry["wilson_max_abs_err_vs_fp64"] = float(np.abs(W - ref["W"]).max())
+                entry["staple_max_abs_err_vs_fp64"] = float(np.abs(Sd - ref["S"]).max())
+                entry["force_max_abs_err_vs_fp64"] = float(np.abs(F - ref["F"]).max())
+            row[st] = entry
+            h.close()
+            s.close()
+            g.close()
+        res[f"eps={eps}"] = row
+    return res
+
+
+def config_c4(klft, a):
+    """64 independent 4D chains, extents from {8,10,12,16,20} per dimension
+    (seeded), U(1)/SU(2) alternating, beta 1.5..2.5, nHits=10, delta=0.1,
+    FP32 links + FP64 plaquette sums: one graph-captured batched sweep vs 64
+    per-chain sweeps."""
+    import numpy as np
+    rng = np.random.default_rng(64)
+    ext = [8, 10, 12, 16, 20]
+    chains = [(1 if k % 2 == 0 else 2, tuple(int(x) for x in rng.choice(ext, 4)), 1.5 + k / 63.0)
+              for k in range(64)]
+    vol = sum(int(np.prod(c[1])) for c in chains)
+    b = klft.Batch(chains, "f32")
+    b.set_identity()
+    b.run(10, 0.1, 7, 0, 3)
+    nsw = 20
+    t = sync_time(lambda: b.run(10, 0.1, 7, 3, nsw))
+    plaq = b.plaquette()
+    fields = [klft.GaugeField(4, nc, dims, "f32").set_identity() for nc, dims, _ in chains]
+    params = [klft.MetropolisParams(nHits=10, beta=beta, delta=0.1, seed=7) for _, _, beta in chains]
+    for f, p in zip(fields, params):
+        klft.metropolis_run(f, p, 0, 1, rates=False)
+
+    def per_chain():
+        for f, p in zip(fields, params):
+            klft.metropolis_run(f, p, 1, nsw, rates=False)
+    t1 = sync_time(per_chain)
+    return {"config": "C4 64 chains, extents {8..20}^4 mixed, U(1)/SU(2), beta 1.5..2.5, FP32",
+            "total_sites": vol, "batched_sweeps_per_s": nsw / t,
+            "batched_link_updates_per_s": 4 * vol * nsw / t, "per_chain_sweeps_per_s": nsw / t1,
+            "per_chain_link_updates_per_s": 4 * vol * nsw / t1, "speedup_batched": t1 / t,
+            "plaquette_min": float(plaq.min()), "plaquette_max": float(plaq.max())}
+
+
+def config_c5(klft, a):
+    """16 SU(3) replicas of 16^4 with a 2x2x2 defect at x0 = 0, c_r = 1 - r/15,
+    nHits=8, delta=0.2, FP32 sweeps, FP64 swap dS; one graph-captured step per
+    replica sweep + swap pass."""
+    import numpy as np
+    n = 16
+    c = 1.0 - np.arange(n) / 15.0
+    p = klft.PTBC(n, 3, (16, 16, 16, 16), 6.0, 2, [0, 0, 0], c, "f32")
+    p.set_identity()
+    for s in range(5):
+        p.step(8, 0.2, 11, s)
+    nst = 40
+    sw = np.zeros(n - 1)
+
+    def run():
+        for s in range(nst):
+            sw[:] += p.step(8, 0.2, 11, 5 + s)
+    t = sync_time(run)
+    return {"config": "C5 PTBC 16 x SU(3) 16^4, defect 2^3 at x0=0, c_r = 1 - r/15, FP32",
+            "steps_per_s": nst / t, "replica_link_updates_per_s": n * 4 * 16 ** 4 * nst / t,
+            "swap_acceptance_per_pair": [float(x) for x in sw / nst],
+            "plaquette_per_replica": [float(x) for x in p.plaquette()], "labels": [int(x) for x in p.labels()]}
+
+
+def run_configs(klft, a):
+    names = ["c1", "c2", "c3", "c4", "c5"] if a.config == "all" else a.config.split(",")
+    fns = {"c1": config_c1, "c2": config_c2, "c3": config_c3, "c4": config_c4, "c5": config_c5}
+    for nm in names:
+        print(f"[bench] config {nm}", file=sys.stderr, flush=True)
+        r = fns[nm](klft, a)
+        print(json.dumps(r, default=float), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -146,6 +371,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import klft_amd as klft
+
+    if a.config:
+        run_configs(klft, a)
+        return
 
     dims = (a.L, a.L, a.L, a.L)
     V = a.L ** 4

@@ -647,6 +647,129 @@ __device__ __forceinline__ void rand_sun(klft_rng::Stream& s, R delta, Mat<R, 3>
 }
 
 // ---------------------------------------------------------------------------
+// One Metropolis hit's draws -- the proposal and the accept uniform that
+// follows it in the hit's stream -- with every Philox block of the hit
+// computed up front.  The blocks are independent, so their rounds interleave
+// instead of forming one long dependent chain (a pass has one wave per SIMD
+// at 32^4, so nothing else hides that latency).  The SU(3) proposal's
+// rejection loops (SUN.hpp:198-233) repeat only when a norm is below 2^-53;
+// then the hit is redrawn by the looping rand_sun above from the start of the
+// same stream, which gives the same values, so the outcome is identical.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct DrawWords {  // stream words per uniform
+  static constexpr int value = sizeof(R) == 8 ? 2 : 1;
+};
+template <int NC>
+struct HitDraws;  // uniforms per hit, the accept draw included
+template <>
+struct HitDraws<1> {
+  static constexpr int value = 2;
+};
+template <>
+struct HitDraws<2> {
+  static constexpr int value = 4;
+};
+template <>
+struct HitDraws<3> {
+  static constexpr int value = 13;
+};
+
+// uniform number i of a hit from its words (Stream::u01 / u01f arithmetic)
+template <typename R>
+__device__ __forceinline__ R u01_at(const uint32_t* w, int i);
+template <>
+__device__ __forceinline__ double u01_at<double>(const uint32_t* w, int i) {
+  const uint32_t a = w[2 * i] >> 5;
+  const uint32_t b = w[2 * i + 1] >> 6;
+  return ((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0);
+}
+template <>
+__device__ __forceinline__ float u01_at<float>(const uint32_t* w, int i) {
+  return (float)(w[i] >> 8) * (1.0f / 16777216.0f);
+}
+template <typename R>
+__device__ __forceinline__ R draw_at(const uint32_t* w, int i, R lo, R hi) {
+  return lo + (hi - lo) * u01_at<R>(w, i);
+}
+
+// first-attempt proposals; false when a rejection loop would repeat
+template <typename R>
+__device__ __forceinline__ bool rand_sun_w(const uint32_t* w, R delta, Mat<R, 1>& r) {
+  const R pi = (R)KLFT_PI;
+  const R th = draw_at<R>(w, 0, -delta * pi, delta * pi);
+  r.m[0][0] = mk(cos(th), sin(th));
+  return true;
+}
+template <typename R>
+__device__ __forceinline__ bool rand_sun_w(const uint32_t* w, R delta, Mat<R, 2>& r) {
+  const R pi = (R)KLFT_PI;
+  const R alpha = draw_at<R>(w, 0, R(0), delta * 2 * pi);
+  const R u = draw_at<R>(w, 1, R(-1), R(1));
+  const R theta = draw_at<R>(w, 2, R(0), R(2) * pi);
+  const R sa = sin(alpha);
+  const R n1 = sqrt(R(1) - u * u) * sin(theta);
+  const R n2 = sqrt(R(1) - u * u) * cos(theta);
+  r.m[0][0] = mk(cos(alpha), u * sa);
+  r.m[0][1] = mk(n1 * sa, n2 * sa);
+  r.m[1][0] = mk(-r.m[0][1].re, r.m[0][1].im);
+  r.m[1][1] = mk(r.m[0][0].re, -r.m[0][0].im);
+  return true;
+}
+template <typename R>
+__device__ __forceinline__ bool rand_sun_w(const uint32_t* w, R delta, Mat<R, 3>& r) {
+  R v[6];
+  cx<R> z1[3], z2[3];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v[i] = draw_at<R>(w, i, R(0), delta);
+  R norm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+  if (!(R(1) != (R(1) + norm))) return false;
+  R fact = R(1) / norm;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) z1[i] = mk(fact * v[2 * i], fact * v[2 * i + 1]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v[i] = draw_at<R>(w, 6 + i, R(0), delta);
+  norm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+  if (!(R(1) != (R(1) + norm))) return false;
+  fact = R(1) / norm;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) z2[i] = mk(fact * v[2 * i], fact * v[2 * i + 1]);
+  const cx<R> z = cmul_(z1[0], z2[0]) + cmul_(z1[1], z2[1]) + cmul_(z1[2], z2[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) z2[i] = z2[i] - z * z1[i];
+  norm = sqrt(z2[0].re * z2[0].re + z2[0].im * z2[0].im + z2[1].re * z2[1].re + z2[1].im * z2[1].im +
+              z2[2].re * z2[2].re + z2[2].im * z2[2].im);
+  if (!(R(1) != (R(1) + norm))) return false;
+  fact = R(1) / norm;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    z2[i] = scale(fact, z2[i]);
+    r.m[0][i] = z1[i];
+    r.m[1][i] = z2[i];
+  }
+  complete_row2(r);
+  return true;
+}
+
+// proposal r and accept uniform u of Metropolis hit `hit` of link (site, mu)
+template <typename R, int NC>
+__device__ __forceinline__ void propose(uint64_t seed, uint32_t site, uint32_t sweep, uint32_t hit,
+                                        uint32_t chain, uint32_t mu, R delta, Mat<R, NC>& r, R& u) {
+  constexpr int NB = (HitDraws<NC>::value * DrawWords<R>::value + 3) / 4;
+  const klft_rng::Stream st(seed, klft_rng::TAG_METROPOLIS, site, sweep, hit, chain, mu);
+  uint32_t w[4 * NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) st.block_words((uint32_t)j, w + 4 * j);
+  if (rand_sun_w(w, delta, r)) {
+    u = u01_at<R>(w, HitDraws<NC>::value - 1);
+    return;
+  }
+  klft_rng::Stream s2 = st;
+  rand_sun(s2, delta, r);
+  u = unif<R>(s2);
+}
+
+// ---------------------------------------------------------------------------
 // Lie algebra: Q = sum_a v_a T_a with T = 1 (U(1)), Pauli sigma (SU(2)),
 // Gell-Mann lambda (SU(3), Tr T_a T_b = 2 delta_ab).
 // ---------------------------------------------------------------------------

@@ -55,12 +55,16 @@ __device__ __forceinline__ Mat<R, NC> load_at(const Links<R, NC, COMP> &f, const
   return u;
 }
 
-// staple with defect-scaled links (same term order as staple())
-template <typename R, int NC, bool COMP, int ND, bool DEFECT>
+// staple with defect-scaled links (same term order as staple()); LOWREG
+// selects the one-term-in-flight staple for a defect-free field
+template <typename R, int NC, bool COMP, int ND, bool DEFECT, bool LOWREG = false>
 __device__ __forceinline__ Mat<R, NC> staple_c(const Links<R, NC, COMP> &f, const Geom &g, const int *x, int mu,
                                                const Defect &d, R c) {
   if constexpr (!DEFECT) {
-    return staple<R, NC, COMP, ND>(f, g, x, mu);
+    if constexpr (LOWREG)
+      return staple_lowreg<R, NC, COMP, ND>(f, g, x, mu);
+    else
+      return staple<R, NC, COMP, ND>(f, g, x, mu);
   } else {
     Mat<R, NC> t = zero<R, NC>();
     int xp[ND], y[ND];
@@ -89,9 +93,27 @@ __device__ __forceinline__ Mat<R, NC> staple_c(const Links<R, NC, COMP> &f, cons
   }
 }
 
+// one Markov step of a link: dS = -(beta/Nc) c_l (Re Tr(r M) - Re Tr M), M = S U
+template <typename R, int NC>
+__device__ __forceinline__ void metro_step(const Mat<R, NC> &r, R u, R beta_nc, R cl, Mat<R, NC> &U,
+                                           Mat<R, NC> &M, R &trM, unsigned long long &n) {
+  const R dS = cl * (-beta_nc * (retr_mul(r, M) - trM));
+  bool accept = dS < R(0);
+  if (!accept) accept = u < exp(-dS);
+  if (accept) {
+    U = mul(U, r);
+    reunitarize(U);
+    M = mul(M, r);  // S U r: the staple is not kept through the hits
+    trM = retrace(M);
+    ++n;
+  }
+}
+
 // update all directions of the site at position t of sublattice `pass`;
-// returns the number of accepted hits
-template <typename R, int NC, bool COMP, int ND, bool DEFECT>
+// returns the number of accepted hits.  HB proposals are drawn before their
+// Markov steps (they do not depend on the field), so their generator rounds
+// overlap; a proposal past nHits is drawn and discarded.
+template <typename R, int NC, bool COMP, int ND, bool DEFECT, bool LOWREG = false, int HB = 1>
 __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP> &f, const Geom &g, int pass,
                                                          long long t, int nhits, R beta_nc, R delta,
                                                          uint64_t seed, uint32_t sweep, uint32_t chain,
@@ -104,28 +126,22 @@ __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP
   if (visit) visit[site] = pass;
 #pragma unroll 1
   for (int mu = 0; mu < ND; ++mu) {
-    const Mat<R, NC> S = staple_c<R, NC, COMP, ND, DEFECT>(f, g, x, mu, d, c);
+    const Mat<R, NC> S = staple_c<R, NC, COMP, ND, DEFECT, LOWREG>(f, g, x, mu, d, c);
     R cl = R(1);
     if constexpr (DEFECT) cl = on_defect<ND>(g, d, x, mu) ? c : R(1);
     Mat<R, NC> U = f.load(s, mu);
-    Mat<R, NC> M = mul(S, U);
+    Mat<R, NC> M = mul(S, U);  // Re Tr(U r S) = Re Tr(r M)
     R trM = retrace(M);
 #pragma unroll 1
-    for (int hit = 0; hit < nhits; ++hit) {
-      klft_rng::Stream st(seed, klft_rng::TAG_METROPOLIS, site, sweep, (uint32_t)hit, chain, (uint32_t)mu);
-      Mat<R, NC> r;
-      rand_sun(st, delta, r);
-      R dS = -beta_nc * (retr_mul(r, M) - trM);
-      if constexpr (DEFECT) dS = cl * dS;
-      bool accept = dS < R(0);
-      if (!accept) accept = unif<R>(st) < exp(-dS);
-      if (accept) {
-        U = mul(U, r);
-        reunitarize(U);
-        M = mul(S, U);
-        trM = retrace(M);
-        ++n;
-      }
+    for (int hit = 0; hit < nhits; hit += HB) {
+      Mat<R, NC> r[HB];
+      R u[HB];
+#pragma unroll
+      for (int k = 0; k < HB; ++k)
+        propose<R, NC>(seed, site, sweep, (uint32_t)(hit + k), chain, (uint32_t)mu, delta, r[k], u[k]);
+#pragma unroll
+      for (int k = 0; k < HB; ++k)
+        if (k == 0 || hit + k < nhits) metro_step(r[k], u[k], beta_nc, cl, U, M, trM, n);
     }
     if (nhits > 0) f.store(s, mu, U);
   }

@@ -9,8 +9,9 @@
 // dS = -(beta/Nc) (Re Tr(U r S) - Re Tr(U S)), accept if dS < 0 or
 // u < exp(-dS) (u drawn only when dS >= 0), on accept U = restoreSUN(U r).
 // The trace is evaluated as Re Tr(r M) with M = S U (Tr is cyclic), so a hit
-// costs one trace instead of the reference's three matrix products; M is
-// rebuilt only after an accepted hit.
+// costs one trace instead of the reference's three matrix products; after an
+// accepted hit M becomes M r, so the staple itself is dead once M is formed
+// (M r and S restoreSUN(U r) differ by rounding only).
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -22,9 +23,13 @@ using namespace kd;
 namespace ki = klft_internal;
 
 constexpr int kMetroBlock = 256;
+constexpr int kMetroVariants = 7;
 
-template <typename R, int NC, bool COMP, int ND>
-__global__ __launch_bounds__(kMetroBlock) void k_metropolis_pass(
+// LOWREG takes the staple one term at a time; HB proposals are drawn ahead of
+// their Markov steps; OCC asks the register allocator for at least OCC waves
+// per SIMD.
+template <typename R, int NC, bool COMP, int ND, bool LOWREG, int HB, int OCC>
+__global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_pass(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
   __shared__ unsigned long long red[kMetroBlock / 64];
@@ -32,8 +37,8 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pass(
   const long long t = (long long)lb * kMetroBlock + threadIdx.x;
   unsigned long long n = 0;
   if (t < g.Vh)
-    n = metro_site<R, NC, COMP, ND, false>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain, Defect{},
-                                           R(1), visit);
+    n = metro_site<R, NC, COMP, ND, false, LOWREG, HB>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep,
+                                                       chain, Defect{}, R(1), visit);
   n = wave_sum_u64(n);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
   __syncthreads();
@@ -118,21 +123,6 @@ __device__ __forceinline__ Mat<R, NC> quad_sum(const Mat<R, NC> &a) {
   return t;
 }
 
-template <typename R, int NC>
-__device__ __forceinline__ void chain_step(const Mat<R, NC> &r, R u, const Mat<R, NC> &S, R beta_nc,
-                                           Mat<R, NC> &U, Mat<R, NC> &M, R &trM, unsigned long long &n) {
-  const R dS = -beta_nc * (retr_mul(r, M) - trM);
-  bool accept = dS < R(0);
-  if (!accept) accept = u < exp(-dS);
-  if (accept) {
-    U = mul(U, r);
-    reunitarize(U);
-    M = mul(S, U);
-    trM = retrace(M);
-    ++n;
-  }
-}
-
 template <typename R, int NC, bool COMP, int ND, int OCC>
 __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_quad(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
@@ -153,24 +143,19 @@ __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_quad(
     if (visit && q == 0) visit[site] = pass;
 #pragma unroll 1
     for (int mu = 0; mu < ND; ++mu) {
-      const Mat<R, NC> S = quad_sum(staple_part<R, NC, COMP, ND>(f, g, x, mu, q));
       Mat<R, NC> U = f.load(s, mu);
-      Mat<R, NC> M = mul(S, U);
+      Mat<R, NC> M = mul(quad_sum(staple_part<R, NC, COMP, ND>(f, g, x, mu, q)), U);
       R trM = retrace(M);
 #pragma unroll 1
       for (int h0 = 0; h0 < nhits; h0 += 4) {
         Mat<R, NC> r = identity<R, NC>();
         R u = R(1);
-        if (h0 + q < nhits) {
-          klft_rng::Stream st(seed, klft_rng::TAG_METROPOLIS, site, sweep, (uint32_t)(h0 + q), chain,
-                              (uint32_t)mu);
-          rand_sun(st, delta, r);
-          u = unif<R>(st);  // the accept draw follows the proposal in the hit's stream
-        }
-        chain_step(bcast<0>(r), bcast_s<0>(u), S, beta_nc, U, M, trM, n);
-        if (h0 + 1 < nhits) chain_step(bcast<1>(r), bcast_s<1>(u), S, beta_nc, U, M, trM, n);
-        if (h0 + 2 < nhits) chain_step(bcast<2>(r), bcast_s<2>(u), S, beta_nc, U, M, trM, n);
-        if (h0 + 3 < nhits) chain_step(bcast<3>(r), bcast_s<3>(u), S, beta_nc, U, M, trM, n);
+        if (h0 + q < nhits)
+          propose<R, NC>(seed, site, sweep, (uint32_t)(h0 + q), chain, (uint32_t)mu, delta, r, u);
+        metro_step(bcast<0>(r), bcast_s<0>(u), beta_nc, R(1), U, M, trM, n);
+        if (h0 + 1 < nhits) metro_step(bcast<1>(r), bcast_s<1>(u), beta_nc, R(1), U, M, trM, n);
+        if (h0 + 2 < nhits) metro_step(bcast<2>(r), bcast_s<2>(u), beta_nc, R(1), U, M, trM, n);
+        if (h0 + 3 < nhits) metro_step(bcast<3>(r), bcast_s<3>(u), beta_nc, R(1), U, M, trM, n);
       }
       if (nhits > 0 && q == 0) f.store(s, mu, U);
     }
@@ -194,7 +179,9 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 namespace klft_internal {
 
 // launch passes [p0, p1) of one sweep on the link array `links` with
-// variant 0 (one lane per site) or 1..3 (quad per site, occupancy 1, 2, 4)
+// variant 0 (one lane per site), 1..3 (quad per site, occupancy 1, 2, 4),
+// 4 (one lane, low-register staple, occupancy 2), 5 (one lane, proposals
+// drawn in pairs) or 6 (one lane, low-register staple, pairs)
 static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p1,
                          const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
                          unsigned long long *counter, int *visit) {
@@ -205,11 +192,11 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
     l.V = f->g.V;
     l.nd = f->nd;
     const R beta_nc = (R)(p->beta / (double)NC);
-    const int lanes = variant >= 1 ? 4 : 1;
-    const unsigned nb = blocks_for(f->g.Vh * lanes, kMetroBlock);
+    const bool quad = variant >= 1 && variant <= 3;
+    const unsigned nb = blocks_for(f->g.Vh * (quad ? 4 : 1), kMetroBlock);
     for (int pass = p0; pass < p1; ++pass) {
       ProfScope prof("metropolis_pass");
-      if (variant >= 1)
+      if (quad)
         with_occ(variant - 1, [&](auto occ) -> int {
           constexpr int OCC = decltype(occ)::value;
           hipLaunchKernelGGL((k_metropolis_quad<R, NC, COMP, ND, OCC>), dim3(nb), dim3(kMetroBlock), 0,
@@ -217,10 +204,18 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
                              (uint32_t)sweep, chain, counter, visit);
           return KLFT_OK;
         });
-      else
-        hipLaunchKernelGGL((k_metropolis_pass<R, NC, COMP, ND>), dim3(nb), dim3(kMetroBlock), 0, stream(), l,
-                           f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed, (uint32_t)sweep, chain,
-                           counter, visit);
+      else {
+        auto one_lane = [&](auto kern) {
+          hipLaunchKernelGGL(kern, dim3(nb), dim3(kMetroBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc,
+                             (R)p->delta, p->seed, (uint32_t)sweep, chain, counter, visit);
+        };
+        switch (variant) {
+          case 4: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 1, 2>); break;
+          case 5: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 2, 1>); break;
+          case 6: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2, 1>); break;
+          default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1, 1>); break;
+        }
+      }
     }
     KLFT_HIP(hipGetLastError());
     return KLFT_OK;
@@ -232,10 +227,13 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
 int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   void *scratch = nullptr;
   // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4
-  // waves/SIMD.  Untuned default (measured at 32^4): the quad split pays off
-  // only for FP32 SU(3), where the proposal generation dominates a hit.
+  // waves/SIMD; 4 one lane with the low-register staple at >= 2 waves/SIMD;
+  // 5, 6 one lane drawing proposals in pairs, with the full / low-register
+  // staple.  Untuned default (measured at 32^4): the quad
+  // split pays off only for FP32 SU(3), where the proposal generation
+  // dominates a hit.
   const int dflt = (f->storage != KLFT_STORAGE_F64 && f->nc == 3) ? 2 : 0;
-  const int v = pick_variant("metropolis", f, 4, dflt, [&](int var) -> int {
+  const int v = pick_variant("metropolis", f, kMetroVariants, dflt, [&](int var) -> int {
     if (!scratch) {
       KLFT_HIP(hipMalloc(&scratch, f->link_bytes));
       KLFT_HIP(hipMemcpyAsync(scratch, f->links, f->link_bytes, hipMemcpyDeviceToDevice, stream()));

@@ -70,6 +70,11 @@ struct Stream {
         hi((tag << 28) | (chain << 18) | (mu << 16)), block(0), b0(0), b1(0), b2(0), b3(0),
         pos(4) {}
 
+  // the four words of block j of this stream (independent of the buffer)
+  KLFT_HD void block_words(uint32_t j, uint32_t o[4]) const {
+    philox4x32_10(c0, c1, c2, hi | (j & 0xffffu), k0, k1, o);
+  }
+
   KLFT_HD void refill() {
     uint32_t o[4];
     philox4x32_10(c0, c1, c2, hi | (block & 0xffffu), k0, k1, o);

@@ -383,3 +383,36 @@ def test_defect_field_and_plaquette():
     assert (cnt1, cnt2) == (24, 12)
     want = nplaq - cnt1 - cnt2 + cnt1 * 0.5 + cnt2 * 0.25
     assert abs(got - want) < 1e-12
+
+
+@pytest.mark.parametrize("nc", [1, 3])
+def test_defect_sweep_without_defect_is_the_plain_sweep(nc):
+    nd, dims = 4, (4, 4, 4, 4)
+    U0 = random_field(nd, nc, dims, 5)
+    Ua, Ub = U0.copy(), U0.copy()
+    ones = np.ones(dims + (nd,))
+    ra = orc.metropolis_sweep(Ua, nd, nc, dims, 4, 2.5, 0.3, 17, 3)
+    rb = orc.metropolis_sweep_defect(Ub, ones, nd, nc, dims, 4, 2.5, 0.3, 17, 3, 0)
+    assert ra == rb
+    assert np.array_equal(Ua, Ub)
+    # another chain index draws another stream
+    Uc = U0.copy()
+    orc.metropolis_sweep_defect(Uc, ones, nd, nc, dims, 4, 2.5, 0.3, 17, 3, 1)
+    assert not np.array_equal(Ua, Uc)
+
+
+def test_defect_links_at_zero_coupling_accept_every_proposal():
+    """A link with c = 0 drops out of the action, so each of its proposals
+    is accepted: after one sweep it equals the same link updated at beta = 0
+    (where every proposal is accepted) from the same start and stream."""
+    nd, nc, dims = 4, 3, (4, 4, 4, 4)
+    C0 = orc.defect_field(nd, dims, 2, [1, 1, 1], 0.0)
+    U0 = random_field(nd, nc, dims, 8)
+    Ud, Uf = U0.copy(), U0.copy()
+    orc.metropolis_sweep_defect(Ud, C0, nd, nc, dims, 3, 6.0, 0.4, 21, 0, 0)
+    orc.metropolis_sweep(Uf, nd, nc, dims, 3, 0.0, 0.4, 21, 0)
+    on = C0 == 0.0
+    assert on.sum() == 8
+    np.testing.assert_allclose(Ud[on], Uf[on], rtol=0, atol=1e-14)
+    # links off the defect at beta = 6 reject some proposals, so they differ
+    assert not np.allclose(Ud[~on], Uf[~on])
