@@ -93,21 +93,55 @@ __device__ __forceinline__ Mat<R, NC> staple_c(const Links<R, NC, COMP> &f, cons
   }
 }
 
-// one Markov step of a link: dS = -(beta/Nc) c_l (Re Tr(r M) - Re Tr M), M = S U
+// The Markov chain of one link through its hits.  dS = -(beta/Nc) c_l
+// (Re Tr(r M) - Re Tr M) with M = S U (Tr is cyclic), so a hit costs one
+// trace; after an accept M becomes M r, so the staple itself is dead once M is
+// formed (M r and S restoreSUN(U r) differ by rounding only).  An accepted
+// proposal is kept in P and multiplied into U (with restoreSUN) only when the
+// next accept comes or the hits end: U goes through exactly the reference's
+// sequence restoreSUN(U r1), restoreSUN(U r2), ... but a wave pays for that
+// work once per link instead of at every hit where any of its 64 lanes
+// accepts (about half the hits at 1 % acceptance).
 template <typename R, int NC>
-__device__ __forceinline__ void metro_step(const Mat<R, NC> &r, R u, R beta_nc, R cl, Mat<R, NC> &U,
-                                           Mat<R, NC> &M, R &trM, unsigned long long &n) {
-  const R dS = cl * (-beta_nc * (retr_mul(r, M) - trM));
-  bool accept = dS < R(0);
-  if (!accept) accept = u < exp(-dS);
-  if (accept) {
-    U = mul(U, r);
-    reunitarize(U);
-    M = mul(M, r);  // S U r: the staple is not kept through the hits
+struct LinkChain {
+  Mat<R, NC> U, M, P;
+  R trM;
+  bool pending;
+
+  __device__ __forceinline__ void start(const Mat<R, NC> &S, const Mat<R, NC> &U0) {
+    U = U0;
+    M = mul(S, U);
     trM = retrace(M);
-    ++n;
+    pending = false;
   }
-}
+  __device__ __forceinline__ void apply_pending() {
+    if (pending) {
+      U = mul(U, P);
+      reunitarize(U);
+      pending = false;
+    }
+  }
+  __device__ __forceinline__ void step(const Mat<R, NC> &r, R u, R beta_nc, R cl, unsigned long long &n) {
+    const R dS = cl * (-beta_nc * (retr_mul(r, M) - trM));
+    bool accept = dS < R(0);
+    if (!accept) {
+      // e^x > 1 + x + x^2/2 + x^3/6 for x > 0, so u * that cubic >= 1 means
+      // u > e^{-dS}: a rejection found without the exponential.  dS > 1/2
+      // keeps the margin (> 0.1 %) far above rounding, so the decision is
+      // always the one u < e^{-dS} gives.
+      const R p3 = R(1) + dS * (R(1) + dS * (R(0.5) + dS * (R(1) / R(6))));
+      if (!(dS > R(0.5) && u * p3 >= R(1))) accept = u < exp(-dS);
+    }
+    if (accept) {
+      apply_pending();
+      P = r;
+      pending = true;
+      M = mul(M, r);
+      trM = retrace(M);
+      ++n;
+    }
+  }
+};
 
 // update all directions of the site at position t of sublattice `pass`;
 // returns the number of accepted hits.  HB proposals are drawn before their
@@ -129,9 +163,8 @@ __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP
     const Mat<R, NC> S = staple_c<R, NC, COMP, ND, DEFECT, LOWREG>(f, g, x, mu, d, c);
     R cl = R(1);
     if constexpr (DEFECT) cl = on_defect<ND>(g, d, x, mu) ? c : R(1);
-    Mat<R, NC> U = f.load(s, mu);
-    Mat<R, NC> M = mul(S, U);  // Re Tr(U r S) = Re Tr(r M)
-    R trM = retrace(M);
+    LinkChain<R, NC> lc;
+    lc.start(S, f.load(s, mu));
 #pragma unroll 1
     for (int hit = 0; hit < nhits; hit += HB) {
       Mat<R, NC> r[HB];
@@ -141,9 +174,120 @@ __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP
         propose<R, NC>(seed, site, sweep, (uint32_t)(hit + k), chain, (uint32_t)mu, delta, r[k], u[k]);
 #pragma unroll
       for (int k = 0; k < HB; ++k)
-        if (k == 0 || hit + k < nhits) metro_step(r[k], u[k], beta_nc, cl, U, M, trM, n);
+        if (k == 0 || hit + k < nhits) lc.step(r[k], u[k], beta_nc, cl, n);
     }
-    if (nhits > 0) f.store(s, mu, U);
+    lc.apply_pending();
+    if (nhits > 0) f.store(s, mu, lc.U);
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Site update with the next direction's staple built during the current
+// direction's hits (a defect-free field).  At 32^4 a pass has one wave per
+// SIMD, so the staple's link loads, issued in bursts between hit phases,
+// leave the memory system idle while the SIMDs compute hits and vice versa.
+// Here hit h of direction mu first issues the three link loads of staple
+// term h of direction m = mu + 1, then runs the hit, then folds the term in,
+// so the loads are in flight while the hit computes.
+//
+// Which links of m's staple can change while mu is updated?  Only U_mu(x)
+// (every other link it reads belongs to another sublattice or was finished
+// earlier in this site's direction loop).  The upper term with nu = mu is
+// therefore split: A = U_mu(x+m) U_m(x+mu)^+ is formed during the hits and
+// A U_mu(x)^+ is added once U_mu(x) is final.  The staple's terms are summed
+// in a different order from staple() (that one term last), so M differs
+// from the one-lane kernel's by rounding only.
+// ---------------------------------------------------------------------------
+template <typename R, int NC, bool COMP, int ND>
+struct TermLoads {
+  Mat<R, NC> a, b, c;
+};
+
+// links of staple term t of link (x, m): t < ND-1 upper (nu = the t-th
+// direction other than m), else lower; the upper term with nu == defer
+// needs no third link
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ void term_load(const Links<R, NC, COMP> &f, const Geom &g, const int *x, long long s,
+                                          int m, int t, int defer, TermLoads<R, NC, COMP, ND> &L) {
+  const bool upper = t < ND - 1;
+  const int j = upper ? t : t - (ND - 1);
+  const int nu = j < m ? j : j + 1;
+  if (upper) {
+    L.a = f.load(idx_shift<ND>(g, x, m, 1, -1, 0), nu);
+    L.b = f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), m);
+    if (nu != defer) L.c = f.load(s, nu);
+  } else {
+    const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+    L.a = f.load(idx_shift<ND>(g, x, m, 1, nu, -1), nu);
+    L.b = f.load(s_xmnu, m);
+    L.c = f.load(s_xmnu, nu);
+  }
+}
+
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ void term_fold(int m, int t, int defer, const TermLoads<R, NC, COMP, ND> &L,
+                                          Mat<R, NC> &acc, Mat<R, NC> &A) {
+  const bool upper = t < ND - 1;
+  const int j = upper ? t : t - (ND - 1);
+  const int nu = j < m ? j : j + 1;
+  if (upper) {
+    const Mat<R, NC> ab = mul_ad(L.a, L.b);
+    if (nu == defer)
+      A = ab;
+    else
+      add_to(acc, mul_ad(ab, L.c));
+  } else {
+    add_to(acc, mul(mul_dd(L.a, L.b), L.c));
+  }
+}
+
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ unsigned long long metro_site_pipe(const Links<R, NC, COMP> &f, const Geom &g,
+                                                              int pass, long long t, int nhits, R beta_nc,
+                                                              R delta, uint64_t seed, uint32_t sweep,
+                                                              uint32_t chain, int *visit) {
+  constexpr int NT = 2 * (ND - 1);
+  unsigned long long n = 0;
+  int x[ND];
+  sub_coords<ND>(g, pass, t, x);
+  const long long s = (long long)pass * g.Vh + t;
+  const uint32_t site = (uint32_t)lex_index<ND>(g, x);
+  if (visit) visit[site] = pass;
+  Mat<R, NC> S = staple_lowreg<R, NC, COMP, ND>(f, g, x, 0);
+#pragma unroll 1
+  for (int mu = 0; mu < ND; ++mu) {
+    LinkChain<R, NC> lc;
+    lc.start(S, f.load(s, mu));
+    const int m = mu + 1;
+    const bool next = m < ND;
+    Mat<R, NC> acc = zero<R, NC>(), A = zero<R, NC>();
+    int k = 0;
+#pragma unroll 1
+    for (int hit = 0; hit < nhits; ++hit) {
+      TermLoads<R, NC, COMP, ND> L;
+      const bool fold = next && k < NT;
+      if (fold) term_load<R, NC, COMP, ND>(f, g, x, s, m, k, mu, L);
+      Mat<R, NC> r;
+      R u;
+      propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
+      lc.step(r, u, beta_nc, R(1), n);
+      if (fold) term_fold<R, NC, COMP, ND>(m, k++, mu, L, acc, A);
+    }
+    if (next) {
+#pragma unroll 1
+      for (; k < NT; ++k) {
+        TermLoads<R, NC, COMP, ND> L;
+        term_load<R, NC, COMP, ND>(f, g, x, s, m, k, mu, L);
+        term_fold<R, NC, COMP, ND>(m, k, mu, L, acc, A);
+      }
+    }
+    lc.apply_pending();
+    if (nhits > 0) f.store(s, mu, lc.U);
+    if (next) {
+      add_to(acc, mul_ad(A, lc.U));  // the upper term through U_mu(x), now final
+      S = acc;
+    }
   }
   return n;
 }

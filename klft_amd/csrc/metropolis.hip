@@ -23,7 +23,7 @@ using namespace kd;
 namespace ki = klft_internal;
 
 constexpr int kMetroBlock = 256;
-constexpr int kMetroVariants = 7;
+constexpr int kMetroVariants = 8;
 
 // LOWREG takes the staple one term at a time; HB proposals are drawn ahead of
 // their Markov steps; OCC asks the register allocator for at least OCC waves
@@ -39,6 +39,28 @@ __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_pass(
   if (t < g.Vh)
     n = metro_site<R, NC, COMP, ND, false, LOWREG, HB>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep,
                                                        chain, Defect{}, R(1), visit);
+  n = wave_sum_u64(n);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+#pragma unroll
+    for (int i = 0; i < kMetroBlock / 64; ++i) b += red[i];
+    if (b) atomicAdd(nacc, b);
+  }
+}
+
+// one lane per site, the next direction's staple loaded during the hits
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe(
+    Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
+    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  __shared__ unsigned long long red[kMetroBlock / 64];
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long t = (long long)lb * kMetroBlock + threadIdx.x;
+  unsigned long long n = 0;
+  if (t < g.Vh)
+    n = metro_site_pipe<R, NC, COMP, ND>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain, visit);
   n = wave_sum_u64(n);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
   __syncthreads();
@@ -143,21 +165,21 @@ __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_quad(
     if (visit && q == 0) visit[site] = pass;
 #pragma unroll 1
     for (int mu = 0; mu < ND; ++mu) {
-      Mat<R, NC> U = f.load(s, mu);
-      Mat<R, NC> M = mul(quad_sum(staple_part<R, NC, COMP, ND>(f, g, x, mu, q)), U);
-      R trM = retrace(M);
+      LinkChain<R, NC> lc;
+      lc.start(quad_sum(staple_part<R, NC, COMP, ND>(f, g, x, mu, q)), f.load(s, mu));
 #pragma unroll 1
       for (int h0 = 0; h0 < nhits; h0 += 4) {
         Mat<R, NC> r = identity<R, NC>();
         R u = R(1);
         if (h0 + q < nhits)
           propose<R, NC>(seed, site, sweep, (uint32_t)(h0 + q), chain, (uint32_t)mu, delta, r, u);
-        metro_step(bcast<0>(r), bcast_s<0>(u), beta_nc, R(1), U, M, trM, n);
-        if (h0 + 1 < nhits) metro_step(bcast<1>(r), bcast_s<1>(u), beta_nc, R(1), U, M, trM, n);
-        if (h0 + 2 < nhits) metro_step(bcast<2>(r), bcast_s<2>(u), beta_nc, R(1), U, M, trM, n);
-        if (h0 + 3 < nhits) metro_step(bcast<3>(r), bcast_s<3>(u), beta_nc, R(1), U, M, trM, n);
+        lc.step(bcast<0>(r), bcast_s<0>(u), beta_nc, R(1), n);
+        if (h0 + 1 < nhits) lc.step(bcast<1>(r), bcast_s<1>(u), beta_nc, R(1), n);
+        if (h0 + 2 < nhits) lc.step(bcast<2>(r), bcast_s<2>(u), beta_nc, R(1), n);
+        if (h0 + 3 < nhits) lc.step(bcast<3>(r), bcast_s<3>(u), beta_nc, R(1), n);
       }
-      if (nhits > 0 && q == 0) f.store(s, mu, U);
+      lc.apply_pending();
+      if (nhits > 0 && q == 0) f.store(s, mu, lc.U);
     }
     if (q != 0) n = 0;
   }
@@ -181,7 +203,8 @@ namespace klft_internal {
 // launch passes [p0, p1) of one sweep on the link array `links` with
 // variant 0 (one lane per site), 1..3 (quad per site, occupancy 1, 2, 4),
 // 4 (one lane, low-register staple, occupancy 2), 5 (one lane, proposals
-// drawn in pairs) or 6 (one lane, low-register staple, pairs)
+// drawn in pairs), 6 (one lane, low-register staple, pairs) or 7 (one lane,
+// next staple loaded during the hits)
 static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p1,
                          const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
                          unsigned long long *counter, int *visit) {
@@ -213,6 +236,7 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
           case 4: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 1, 2>); break;
           case 5: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 2, 1>); break;
           case 6: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2, 1>); break;
+          case 7: one_lane(k_metropolis_pipe<R, NC, COMP, ND>); break;
           default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1, 1>); break;
         }
       }
@@ -229,7 +253,7 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4
   // waves/SIMD; 4 one lane with the low-register staple at >= 2 waves/SIMD;
   // 5, 6 one lane drawing proposals in pairs, with the full / low-register
-  // staple.  Untuned default (measured at 32^4): the quad
+  // staple; 7 one lane building the next direction's staple during the hits.  Untuned default (measured at 32^4): the quad
   // split pays off only for FP32 SU(3), where the proposal generation
   // dominates a hit.
   const int dflt = (f->storage != KLFT_STORAGE_F64 && f->nc == 3) ? 2 : 0;
