@@ -194,10 +194,9 @@ __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP
 // Which links of m's staple can change while mu is updated?  Only U_mu(x)
 // (every other link it reads belongs to another sublattice or was finished
 // earlier in this site's direction loop).  The upper term with nu = mu is
-// therefore split: A = U_mu(x+m) U_m(x+mu)^+ is formed during the hits and
-// A U_mu(x)^+ is added once U_mu(x) is final.  The staple's terms are summed
-// in a different order from staple() (that one term last), so M differs
-// from the one-lane kernel's by rounding only.
+// therefore formed after the hits, from the final U_mu(x).  The staple's terms
+// are summed in a different order from staple() (that one term last), so M
+// differs from the one-lane kernel's by rounding only.
 // ---------------------------------------------------------------------------
 template <typename R, int NC, bool COMP, int ND>
 struct TermLoads {
@@ -205,8 +204,8 @@ struct TermLoads {
 };
 
 // links of staple term t of link (x, m): t < ND-1 upper (nu = the t-th
-// direction other than m), else lower; the upper term with nu == defer
-// needs no third link
+// direction other than m), else lower; the upper term with nu == defer is
+// skipped here
 template <typename R, int NC, bool COMP, int ND>
 __device__ __forceinline__ void term_load(const Links<R, NC, COMP> &f, const Geom &g, const int *x, long long s,
                                           int m, int t, int defer, TermLoads<R, NC, COMP, ND> &L) {
@@ -214,9 +213,10 @@ __device__ __forceinline__ void term_load(const Links<R, NC, COMP> &f, const Geo
   const int j = upper ? t : t - (ND - 1);
   const int nu = j < m ? j : j + 1;
   if (upper) {
+    if (nu == defer) return;
     L.a = f.load(idx_shift<ND>(g, x, m, 1, -1, 0), nu);
     L.b = f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), m);
-    if (nu != defer) L.c = f.load(s, nu);
+    L.c = f.load(s, nu);
   } else {
     const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
     L.a = f.load(idx_shift<ND>(g, x, m, 1, nu, -1), nu);
@@ -227,16 +227,12 @@ __device__ __forceinline__ void term_load(const Links<R, NC, COMP> &f, const Geo
 
 template <typename R, int NC, bool COMP, int ND>
 __device__ __forceinline__ void term_fold(int m, int t, int defer, const TermLoads<R, NC, COMP, ND> &L,
-                                          Mat<R, NC> &acc, Mat<R, NC> &A) {
+                                          Mat<R, NC> &acc) {
   const bool upper = t < ND - 1;
   const int j = upper ? t : t - (ND - 1);
   const int nu = j < m ? j : j + 1;
   if (upper) {
-    const Mat<R, NC> ab = mul_ad(L.a, L.b);
-    if (nu == defer)
-      A = ab;
-    else
-      add_to(acc, mul_ad(ab, L.c));
+    if (nu != defer) add_to(acc, mul_ad(mul_ad(L.a, L.b), L.c));
   } else {
     add_to(acc, mul(mul_dd(L.a, L.b), L.c));
   }
@@ -261,7 +257,7 @@ __device__ __forceinline__ unsigned long long metro_site_pipe(const Links<R, NC,
     lc.start(S, f.load(s, mu));
     const int m = mu + 1;
     const bool next = m < ND;
-    Mat<R, NC> acc = zero<R, NC>(), A = zero<R, NC>();
+    Mat<R, NC> acc = zero<R, NC>();
     int k = 0;
 #pragma unroll 1
     for (int hit = 0; hit < nhits; ++hit) {
@@ -272,20 +268,23 @@ __device__ __forceinline__ unsigned long long metro_site_pipe(const Links<R, NC,
       R u;
       propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
       lc.step(r, u, beta_nc, R(1), n);
-      if (fold) term_fold<R, NC, COMP, ND>(m, k++, mu, L, acc, A);
+      if (fold) term_fold<R, NC, COMP, ND>(m, k++, mu, L, acc);
     }
     if (next) {
 #pragma unroll 1
       for (; k < NT; ++k) {
         TermLoads<R, NC, COMP, ND> L;
         term_load<R, NC, COMP, ND>(f, g, x, s, m, k, mu, L);
-        term_fold<R, NC, COMP, ND>(m, k, mu, L, acc, A);
+        term_fold<R, NC, COMP, ND>(m, k, mu, L, acc);
       }
     }
     lc.apply_pending();
     if (nhits > 0) f.store(s, mu, lc.U);
     if (next) {
-      add_to(acc, mul_ad(A, lc.U));  // the upper term through U_mu(x), now final
+      // the upper term through U_mu(x), now final
+      const Mat<R, NC> ab = mul_ad(f.load(idx_shift<ND>(g, x, m, 1, -1, 0), mu),
+                                   f.load(idx_shift<ND>(g, x, mu, 1, -1, 0), m));
+      add_to(acc, mul_ad(ab, lc.U));
       S = acc;
     }
   }

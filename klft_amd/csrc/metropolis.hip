@@ -253,10 +253,13 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4
   // waves/SIMD; 4 one lane with the low-register staple at >= 2 waves/SIMD;
   // 5, 6 one lane drawing proposals in pairs, with the full / low-register
-  // staple; 7 one lane building the next direction's staple during the hits.  Untuned default (measured at 32^4): the quad
-  // split pays off only for FP32 SU(3), where the proposal generation
-  // dominates a hit.
-  const int dflt = (f->storage != KLFT_STORAGE_F64 && f->nc == 3) ? 2 : 0;
+  // staple; 7 one lane building the next direction's staple during the hits.
+  // Untuned defaults, the fastest at 32^4 (nHits = 10): one lane for SU(3)
+  // and FP64 U(1); pipelined staple for FP64 SU(2); paired proposals for
+  // FP32 SU(2) and U(1).
+  int dflt = 0;
+  if (f->nc == 2) dflt = f->storage == KLFT_STORAGE_F64 ? 7 : 5;
+  if (f->nc == 1 && f->storage != KLFT_STORAGE_F64) dflt = 5;
   const int v = pick_variant("metropolis", f, kMetroVariants, dflt, [&](int var) -> int {
     if (!scratch) {
       KLFT_HIP(hipMalloc(&scratch, f->link_bytes));
