@@ -211,36 +211,48 @@ def config_c1(klft, a):
     return out
 
 
-def config_c2(klft, a):
-    """SU(2) 16^4 HMC, beta 2.3, Haar hot start, tau 1, 20 steps, 200
-    trajectories: FP64 links vs FP32 links (+ FP64 dH and action sums)."""
+def _hmc_chain(klft, st, nsteps, ntraj, therm):
+    """One SU(2) 16^4 HMC chain (beta 2.3, tau 1, Haar hot start): rates,
+    acceptance, binned plaquette and <exp(-dH)> after `therm` trajectories."""
     import numpy as np
-    res = {"config": "C2 SU(2) 16^4 HMC beta=2.3 tau=1 n=20, 200 trajectories"}
+    g = klft.GaugeField(4, 2, (16, 16, 16, 16), st).set_haar(2 * 1234)
+    h = klft.HMC(g, beta=2.3, tau=1.0, nsteps=nsteps, seed=1234)
+    dH, acc, plaq = [], [], []
+    t0 = time.perf_counter()
+    for i in range(ntraj):
+        d, ok = h.trajectory(i)
+        dH.append(d)
+        acc.append(ok)
+        plaq.append(klft.GaugePlaquette(g))
+    t = time.perf_counter() - t0
+    e = np.exp(-np.array(dH[therm:]))
+    p = np.array(plaq[therm:])
+    nb = 10
+    eb = np.array([b.mean() for b in np.array_split(e, nb)])
+    pb = np.array([b.mean() for b in np.array_split(p, nb)])
+    h.close()
+    g.close()
+    return {"traj_per_s": ntraj / t, "acceptance": float(np.mean(acc[therm:])),
+            "plaquette": float(p.mean()), "plaquette_err": float(pb.std(ddof=1) / np.sqrt(nb)),
+            "exp_minus_dH": float(e.mean()), "exp_minus_dH_err": float(eb.std(ddof=1) / np.sqrt(nb)),
+            "mean_dH": float(np.mean(dH[therm:])), "mean_abs_dH": float(np.abs(dH[therm:]).mean())}
+
+
+def config_c2(klft, a):
+    """SU(2) 16^4 HMC, beta 2.3, Haar hot start, tau 1: the configured 20
+    steps (21 gauge updates, 200 trajectories) with FP64 and with FP32 links
+    (+ FP64 dH and action sums), and 80 steps (400 trajectories), where the
+    step is small enough for <exp(-dH)> to be measured with a useful error.
+    The generators are the reference's sigma_a (Tr T_a T_b = 2 delta_ab), so
+    a step moves twice as far as with sigma_a / 2."""
+    import numpy as np
+    res = {"config": "C2 SU(2) 16^4 HMC beta=2.3 tau=1, n=20 (200 traj) and n=80 (400 traj)"}
     for st in ["f64", "f32"]:
-        g = klft.GaugeField(4, 2, (16, 16, 16, 16), st).set_haar(2 * 1234)
-        h = klft.HMC(g, beta=2.3, tau=1.0, nsteps=20, seed=1234)
-        dH, acc, plaq = [], [], []
-        t0 = time.perf_counter()
-        for i in range(200):
-            d, ok = h.trajectory(i)
-            dH.append(d)
-            acc.append(ok)
-            plaq.append(klft.GaugePlaquette(g))
-        t = time.perf_counter() - t0
-        therm = 50
-        e = np.exp(-np.array(dH[therm:]))
-        p = np.array(plaq[therm:])
-        nb = 10
-        eb = np.array([b.mean() for b in np.array_split(e, nb)])
-        pb = np.array([b.mean() for b in np.array_split(p, nb)])
-        res[st] = {"traj_per_s": 200 / t, "acceptance": float(np.mean(acc)),
-                   "plaquette": float(p.mean()), "plaquette_err": float(pb.std(ddof=1) / np.sqrt(nb)),
-                   "exp_minus_dH": float(e.mean()), "exp_minus_dH_err": float(eb.std(ddof=1) / np.sqrt(nb)),
-                   "mean_abs_dH": float(np.abs(dH[therm:]).mean())}
-        h.close()
-        g.close()
+        res[st] = _hmc_chain(klft, st, 20, 200, 50)
     d = res["f32"]["plaquette"] - res["f64"]["plaquette"]
     res["plaquette_fp32_minus_fp64_sigma"] = d / np.hypot(res["f32"]["plaquette_err"], res["f64"]["plaquette_err"])
+    for st in ["f64", "f32"]:
+        res[st + "_n80"] = _hmc_chain(klft, st, 80, 400, 100)
     return res
 
 
