@@ -23,7 +23,7 @@ using namespace kd;
 namespace ki = klft_internal;
 
 constexpr int kMetroBlock = 256;
-constexpr int kMetroVariants = 8;
+constexpr int kMetroVariants = 9;
 
 // LOWREG takes the staple one term at a time; HB proposals are drawn ahead of
 // their Markov steps; OCC asks the register allocator for at least OCC waves
@@ -61,6 +61,32 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe(
   unsigned long long n = 0;
   if (t < g.Vh)
     n = metro_site_pipe<R, NC, COMP, ND>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain, visit);
+  n = wave_sum_u64(n);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+#pragma unroll
+    for (int i = 0; i < kMetroBlock / 64; ++i) b += red[i];
+    if (b) atomicAdd(nacc, b);
+  }
+}
+
+// the same with the next staple's links staged through LDS (FP64 links)
+template <int NC, int ND>
+__global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe_lds(
+    Links<double, NC, false> f, Geom g, int pass, int nhits, double beta_nc, double delta, uint64_t seed,
+    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  __shared__ unsigned long long red[kMetroBlock / 64];
+  __shared__ cx<double> stage[kMetroBlock / 64][3 * NC * NC * 64];
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long t = (long long)lb * kMetroBlock + threadIdx.x;
+  unsigned long long n = 0;
+  // each lane's LDS transfers land in, and are read from, its own slots of
+  // the wave's region, so lanes past Vh (which issue none) disturb nothing
+  if (t < g.Vh)
+    n = metro_site_pipe<double, NC, false, ND, true>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain,
+                                                     visit, stage[threadIdx.x >> 6]);
   n = wave_sum_u64(n);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
   __syncthreads();
@@ -203,8 +229,8 @@ namespace klft_internal {
 // launch passes [p0, p1) of one sweep on the link array `links` with
 // variant 0 (one lane per site), 1..3 (quad per site, occupancy 1, 2, 4),
 // 4 (one lane, low-register staple, occupancy 2), 5 (one lane, proposals
-// drawn in pairs), 6 (one lane, low-register staple, pairs) or 7 (one lane,
-// next staple loaded during the hits)
+// drawn in pairs), 6 (one lane, low-register staple, pairs), 7 (one lane,
+// next staple loaded during the hits) or 8 (as 7, FP64 links staged in LDS)
 static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p1,
                          const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
                          unsigned long long *counter, int *visit) {
@@ -237,6 +263,12 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
           case 5: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 2, 1>); break;
           case 6: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2, 1>); break;
           case 7: one_lane(k_metropolis_pipe<R, NC, COMP, ND>); break;
+          case 8:
+            if constexpr (std::is_same<R, double>::value && !COMP)
+              one_lane(k_metropolis_pipe_lds<NC, ND>);
+            else
+              one_lane(k_metropolis_pipe<R, NC, COMP, ND>);
+            break;
           default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1, 1>); break;
         }
       }
@@ -253,7 +285,8 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4
   // waves/SIMD; 4 one lane with the low-register staple at >= 2 waves/SIMD;
   // 5, 6 one lane drawing proposals in pairs, with the full / low-register
-  // staple; 7 one lane building the next direction's staple during the hits.
+  // staple; 7 one lane building the next direction's staple during the hits;
+  // 8 the same with FP64 links staged through LDS (7 for FP32 storage).
   // Untuned defaults, the fastest at 32^4 (nHits = 10): one lane for SU(3)
   // and FP64 U(1); pipelined staple for FP64 SU(2); paired proposals for
   // FP32 SU(2) and U(1).
