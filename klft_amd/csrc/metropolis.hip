@@ -23,7 +23,7 @@ using namespace kd;
 namespace ki = klft_internal;
 
 constexpr int kMetroBlock = 256;
-constexpr int kMetroVariants = 9;
+constexpr int kMetroVariants = 11;
 
 // LOWREG takes the staple one term at a time; HB proposals are drawn ahead of
 // their Markov steps; OCC asks the register allocator for at least OCC waves
@@ -99,15 +99,16 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe_lds(
 }
 
 // ---------------------------------------------------------------------------
-// Quad variant: four lanes per site.  The proposals (and their accept draws)
-// do not depend on the field, so for hits h0..h0+3 lane q generates the
-// proposal of hit h0+q from its own keyed stream; the quad then steps the
-// Markov chain through the four hits in order, each proposal broadcast from
-// its lane with a DPP quad permute, every lane computing the same decision
-// and the same updated link.  The 2(nd-1) staple terms are split over the
-// lanes (lane q: terms q and q+4) and summed with two commutative DPP
-// exchanges, so all lanes hold the bitwise-identical staple.  Four times the
-// threads of the one-lane kernel and a quarter of its generator work per lane.
+// Group variants: four (quad) or two (pair) lanes per site.  The proposals
+// (and their accept draws) do not depend on the field, so for hits
+// h0..h0+LANES-1 lane q generates the proposal of hit h0+q from its own keyed
+// stream; the group then steps the Markov chain through those hits in order,
+// each proposal broadcast from its lane with a DPP quad permute, every lane
+// computing the same decision and the same updated link.  The 2(nd-1) staple
+// terms are split over the lanes (lane q: terms q, q+LANES, ...) and summed
+// with commutative DPP exchanges, so all lanes hold the bitwise-identical
+// staple.  LANES times the waves of the one-lane kernel (a 32^4 pass has only
+// one wave per SIMD otherwise) and 1/LANES of its generator work per lane.
 // ---------------------------------------------------------------------------
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
@@ -129,24 +130,29 @@ __device__ __forceinline__ Mat<R, NC> dpp_mat(const Mat<R, NC> &a) {
     for (int j = 0; j < NC; ++j) b.m[i][j] = mk(dpp<CTRL>(a.m[i][j].re), dpp<CTRL>(a.m[i][j].im));
   return b;
 }
-// quad_perm broadcast of lane J: [J, J, J, J]
-template <int J, typename R, int NC>
+// broadcast of lane J of each group of LANES lanes: quad_perm [J, J, J, J]
+// for quads, [J, J, 2+J, 2+J] for pairs
+template <int LANES, int J>
+struct BcastCtrl {
+  static constexpr int value = LANES == 4 ? J * 0x55 : (J == 0 ? 0xA0 : 0xF5);
+};
+template <int LANES, int J, typename R, int NC>
 __device__ __forceinline__ Mat<R, NC> bcast(const Mat<R, NC> &a) {
-  return dpp_mat<J * 0x55>(a);
+  return dpp_mat<BcastCtrl<LANES, J>::value>(a);
 }
-template <int J, typename R>
+template <int LANES, int J, typename R>
 __device__ __forceinline__ R bcast_s(R v) {
-  return dpp<J * 0x55>(v);
+  return dpp<BcastCtrl<LANES, J>::value>(v);
 }
 
-// partial staple of lane q: terms q and q + 4 of [up(nu_0..), low(nu_0..)]
-template <typename R, int NC, bool COMP, int ND>
+// partial staple of lane q: terms q, q + LANES, ... of [up(nu_0..), low(nu_0..)]
+template <typename R, int NC, bool COMP, int ND, int LANES>
 __device__ __forceinline__ Mat<R, NC> staple_part(const Links<R, NC, COMP> &f, const Geom &g,
                                                   const int *x, int mu, int q) {
   constexpr int NT = 2 * (ND - 1);
   Mat<R, NC> t = zero<R, NC>();
 #pragma unroll 1
-  for (int k = q; k < NT; k += 4) {
+  for (int k = q; k < NT; k += LANES) {
     const bool upper = k < ND - 1;
     const int j = upper ? k : k - (ND - 1);
     const int nu = j < mu ? j : j + 1;  // j-th direction other than mu
@@ -163,23 +169,27 @@ __device__ __forceinline__ Mat<R, NC> staple_part(const Links<R, NC, COMP> &f, c
   return t;
 }
 
-template <typename R, int NC>
-__device__ __forceinline__ Mat<R, NC> quad_sum(const Mat<R, NC> &a) {
+// sum over the group's lanes; each exchange is commutative, so every lane
+// of the group holds the bitwise-identical sum
+template <int LANES, typename R, int NC>
+__device__ __forceinline__ Mat<R, NC> group_sum(const Mat<R, NC> &a) {
   Mat<R, NC> t = a;
-  add_to(t, dpp_mat<0xB1>(t));  // quad_perm [1,0,3,2]
-  add_to(t, dpp_mat<0x4E>(t));  // quad_perm [2,3,0,1]
+  add_to(t, dpp_mat<0xB1>(t));                           // quad_perm [1,0,3,2]
+  if constexpr (LANES == 4) add_to(t, dpp_mat<0x4E>(t));  // quad_perm [2,3,0,1]
   return t;
 }
 
-template <typename R, int NC, bool COMP, int ND, int OCC>
-__global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_quad(
+// LANES = 4 (quad) or 2 (pair) lanes per site
+template <typename R, int NC, bool COMP, int ND, int OCC, int LANES>
+__global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_group(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  static_assert(LANES == 2 || LANES == 4, "two or four lanes per site");
   __shared__ unsigned long long red[kMetroBlock / 64];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
-  const long long t = ((long long)lb * kMetroBlock + threadIdx.x) >> 2;
-  const int q = threadIdx.x & 3;
-  // a whole quad is either in range or not, so the DPP exchanges never read
+  const long long t = ((long long)lb * kMetroBlock + threadIdx.x) / LANES;
+  const int q = threadIdx.x & (LANES - 1);
+  // a whole group is either in range or not, so the DPP exchanges never read
   // an inactive lane
   const bool active = t < g.Vh;
   unsigned long long n = 0;
@@ -192,17 +202,19 @@ __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_quad(
 #pragma unroll 1
     for (int mu = 0; mu < ND; ++mu) {
       LinkChain<R, NC> lc;
-      lc.start(quad_sum(staple_part<R, NC, COMP, ND>(f, g, x, mu, q)), f.load(s, mu));
+      lc.start(group_sum<LANES>(staple_part<R, NC, COMP, ND, LANES>(f, g, x, mu, q)), f.load(s, mu));
 #pragma unroll 1
-      for (int h0 = 0; h0 < nhits; h0 += 4) {
+      for (int h0 = 0; h0 < nhits; h0 += LANES) {
         Mat<R, NC> r = identity<R, NC>();
         R u = R(1);
         if (h0 + q < nhits)
           propose<R, NC>(seed, site, sweep, (uint32_t)(h0 + q), chain, (uint32_t)mu, delta, r, u);
-        lc.step(bcast<0>(r), bcast_s<0>(u), beta_nc, R(1), n);
-        if (h0 + 1 < nhits) lc.step(bcast<1>(r), bcast_s<1>(u), beta_nc, R(1), n);
-        if (h0 + 2 < nhits) lc.step(bcast<2>(r), bcast_s<2>(u), beta_nc, R(1), n);
-        if (h0 + 3 < nhits) lc.step(bcast<3>(r), bcast_s<3>(u), beta_nc, R(1), n);
+        lc.step(bcast<LANES, 0>(r), bcast_s<LANES, 0>(u), beta_nc, R(1), n);
+        if (h0 + 1 < nhits) lc.step(bcast<LANES, 1>(r), bcast_s<LANES, 1>(u), beta_nc, R(1), n);
+        if constexpr (LANES == 4) {
+          if (h0 + 2 < nhits) lc.step(bcast<LANES, 2>(r), bcast_s<LANES, 2>(u), beta_nc, R(1), n);
+          if (h0 + 3 < nhits) lc.step(bcast<LANES, 3>(r), bcast_s<LANES, 3>(u), beta_nc, R(1), n);
+        }
       }
       lc.apply_pending();
       if (nhits > 0 && q == 0) f.store(s, mu, lc.U);
@@ -230,7 +242,8 @@ namespace klft_internal {
 // variant 0 (one lane per site), 1..3 (quad per site, occupancy 1, 2, 4),
 // 4 (one lane, low-register staple, occupancy 2), 5 (one lane, proposals
 // drawn in pairs), 6 (one lane, low-register staple, pairs), 7 (one lane,
-// next staple loaded during the hits) or 8 (as 7, FP64 links staged in LDS)
+// next staple loaded during the hits), 8 (as 7, FP64 links staged in LDS)
+// or 9, 10 (pair per site, occupancy 2, 4)
 static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p1,
                          const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
                          unsigned long long *counter, int *visit) {
@@ -242,13 +255,22 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
     l.nd = f->nd;
     const R beta_nc = (R)(p->beta / (double)NC);
     const bool quad = variant >= 1 && variant <= 3;
-    const unsigned nb = blocks_for(f->g.Vh * (quad ? 4 : 1), kMetroBlock);
+    const bool pair = variant == 9 || variant == 10;
+    const unsigned nb = blocks_for(f->g.Vh * (quad ? 4 : (pair ? 2 : 1)), kMetroBlock);
     for (int pass = p0; pass < p1; ++pass) {
       ProfScope prof("metropolis_pass");
       if (quad)
         with_occ(variant - 1, [&](auto occ) -> int {
           constexpr int OCC = decltype(occ)::value;
-          hipLaunchKernelGGL((k_metropolis_quad<R, NC, COMP, ND, OCC>), dim3(nb), dim3(kMetroBlock), 0,
+          hipLaunchKernelGGL((k_metropolis_group<R, NC, COMP, ND, OCC, 4>), dim3(nb), dim3(kMetroBlock), 0,
+                             stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
+                             (uint32_t)sweep, chain, counter, visit);
+          return KLFT_OK;
+        });
+      else if (pair)
+        with_occ(variant - 8, [&](auto occ) -> int {  // 9: >= 2, 10: >= 4 waves/SIMD
+          constexpr int OCC = decltype(occ)::value;
+          hipLaunchKernelGGL((k_metropolis_group<R, NC, COMP, ND, OCC, 2>), dim3(nb), dim3(kMetroBlock), 0,
                              stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
                              (uint32_t)sweep, chain, counter, visit);
           return KLFT_OK;
@@ -286,7 +308,8 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   // waves/SIMD; 4 one lane with the low-register staple at >= 2 waves/SIMD;
   // 5, 6 one lane drawing proposals in pairs, with the full / low-register
   // staple; 7 one lane building the next direction's staple during the hits;
-  // 8 the same with FP64 links staged through LDS (7 for FP32 storage).
+  // 8 the same with FP64 links staged through LDS (7 for FP32 storage);
+  // 9, 10 a pair of lanes per site with >= 2, 4 waves/SIMD.
   // Untuned defaults, the fastest at 32^4 (nHits = 10): one lane for SU(3)
   // and FP64 U(1); pipelined staple for FP64 SU(2); paired proposals for
   // FP32 SU(2) and U(1).
