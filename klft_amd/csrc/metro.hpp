@@ -183,6 +183,73 @@ __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP
 }
 
 // ---------------------------------------------------------------------------
+// Site update keeping the site's own links in registers (a defect-free
+// field).  The staple of U_mu(x) reads U_nu(x) for every nu != mu, so the
+// one-lane kernel loads each own link four times per pass (once to update
+// it, three times in other directions' staples), 16 of its 76 link loads
+// per site; here they are loaded once.  The direction loop is unrolled so
+// the links stay in named registers.
+// ---------------------------------------------------------------------------
+template <typename R, int NC, bool COMP, int ND, int MU>
+__device__ __forceinline__ void metro_dir_own(const Links<R, NC, COMP> &f, const Geom &g, const int *x,
+                                              long long s, uint32_t site, int nhits, R beta_nc, R delta,
+                                              uint64_t seed, uint32_t sweep, uint32_t chain, Mat<R, NC> *Uown,
+                                              unsigned long long &n) {
+  Mat<R, NC> S = zero<R, NC>();
+  const long long s_xpmu = idx_shift<ND>(g, x, MU, 1, -1, 0);
+#pragma unroll
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == MU) continue;
+    const Mat<R, NC> ab = mul_ad(f.load(s_xpmu, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), MU));
+    add_to(S, mul_ad(ab, Uown[nu]));
+  }
+#pragma unroll 1
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == MU) continue;
+    const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+    const Mat<R, NC> ab = mul_dd(f.load(idx_shift<ND>(g, x, MU, 1, nu, -1), nu), f.load(s_xmnu, MU));
+    add_to(S, mul(ab, f.load(s_xmnu, nu)));
+  }
+  LinkChain<R, NC> lc;
+  lc.start(S, Uown[MU]);
+#pragma unroll 1
+  for (int hit = 0; hit < nhits; ++hit) {
+    Mat<R, NC> r;
+    R u;
+    propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)MU, delta, r, u);
+    lc.step(r, u, beta_nc, R(1), n);
+  }
+  lc.apply_pending();
+  if (nhits > 0) {
+    Uown[MU] = lc.U;
+    f.store(s, MU, lc.U);
+  }
+}
+
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ unsigned long long metro_site_own(const Links<R, NC, COMP> &f, const Geom &g,
+                                                             int pass, long long t, int nhits, R beta_nc,
+                                                             R delta, uint64_t seed, uint32_t sweep,
+                                                             uint32_t chain, int *visit) {
+  unsigned long long n = 0;
+  int x[ND];
+  sub_coords<ND>(g, pass, t, x);
+  const long long s = (long long)pass * g.Vh + t;
+  const uint32_t site = (uint32_t)lex_index<ND>(g, x);
+  if (visit) visit[site] = pass;
+  Mat<R, NC> Uown[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) Uown[d] = f.load(s, d);
+  metro_dir_own<R, NC, COMP, ND, 0>(f, g, x, s, site, nhits, beta_nc, delta, seed, sweep, chain, Uown, n);
+  metro_dir_own<R, NC, COMP, ND, 1>(f, g, x, s, site, nhits, beta_nc, delta, seed, sweep, chain, Uown, n);
+  if constexpr (ND > 2)
+    metro_dir_own<R, NC, COMP, ND, 2>(f, g, x, s, site, nhits, beta_nc, delta, seed, sweep, chain, Uown, n);
+  if constexpr (ND > 3)
+    metro_dir_own<R, NC, COMP, ND, 3>(f, g, x, s, site, nhits, beta_nc, delta, seed, sweep, chain, Uown, n);
+  return n;
+}
+
+// ---------------------------------------------------------------------------
 // Site update with the next direction's staple built during the current
 // direction's hits (a defect-free field).  At 32^4 a pass has one wave per
 // SIMD, so the staple's link loads, issued in bursts between hit phases,

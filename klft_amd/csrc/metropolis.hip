@@ -23,7 +23,7 @@ using namespace kd;
 namespace ki = klft_internal;
 
 constexpr int kMetroBlock = 256;
-constexpr int kMetroVariants = 11;
+constexpr int kMetroVariants = 12;
 
 // LOWREG takes the staple one term at a time; HB proposals are drawn ahead of
 // their Markov steps; OCC asks the register allocator for at least OCC waves
@@ -61,6 +61,28 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe(
   unsigned long long n = 0;
   if (t < g.Vh)
     n = metro_site_pipe<R, NC, COMP, ND>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain, visit);
+  n = wave_sum_u64(n);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+#pragma unroll
+    for (int i = 0; i < kMetroBlock / 64; ++i) b += red[i];
+    if (b) atomicAdd(nacc, b);
+  }
+}
+
+// one lane per site, the site's own links kept in registers
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(kMetroBlock) void k_metropolis_own(
+    Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
+    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  __shared__ unsigned long long red[kMetroBlock / 64];
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long t = (long long)lb * kMetroBlock + threadIdx.x;
+  unsigned long long n = 0;
+  if (t < g.Vh)
+    n = metro_site_own<R, NC, COMP, ND>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain, visit);
   n = wave_sum_u64(n);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
   __syncthreads();
@@ -243,7 +265,8 @@ namespace klft_internal {
 // 4 (one lane, low-register staple, occupancy 2), 5 (one lane, proposals
 // drawn in pairs), 6 (one lane, low-register staple, pairs), 7 (one lane,
 // next staple loaded during the hits), 8 (as 7, FP64 links staged in LDS)
-// or 9, 10 (pair per site, occupancy 2, 4)
+// 9, 10 (pair per site, occupancy 2, 4) or 11 (one lane, own links kept
+// in registers)
 static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p1,
                          const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
                          unsigned long long *counter, int *visit) {
@@ -285,6 +308,7 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
           case 5: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 2, 1>); break;
           case 6: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2, 1>); break;
           case 7: one_lane(k_metropolis_pipe<R, NC, COMP, ND>); break;
+          case 11: one_lane(k_metropolis_own<R, NC, COMP, ND>); break;
           case 8:
             if constexpr (std::is_same<R, double>::value && !COMP)
               one_lane(k_metropolis_pipe_lds<NC, ND>);
@@ -309,13 +333,11 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   // 5, 6 one lane drawing proposals in pairs, with the full / low-register
   // staple; 7 one lane building the next direction's staple during the hits;
   // 8 the same with FP64 links staged through LDS (7 for FP32 storage);
-  // 9, 10 a pair of lanes per site with >= 2, 4 waves/SIMD.
-  // Untuned defaults, the fastest at 32^4 (nHits = 10): one lane for SU(3)
-  // and FP64 U(1); pipelined staple for FP64 SU(2); paired proposals for
-  // FP32 SU(2) and U(1).
-  int dflt = 0;
-  if (f->nc == 2) dflt = f->storage == KLFT_STORAGE_F64 ? 7 : 5;
-  if (f->nc == 1 && f->storage != KLFT_STORAGE_F64) dflt = 5;
+  // 9, 10 a pair of lanes per site with >= 2, 4 waves/SIMD; 11 one lane
+  // with the site's own links kept in registers.
+  // Untuned defaults, the fastest at 32^4 (nHits = 10): LDS-staged
+  // pipelined staple for FP64 SU(3), own links in registers otherwise.
+  const int dflt = (f->nc == 3 && f->storage == KLFT_STORAGE_F64) ? 8 : 11;
   const int v = pick_variant("metropolis", f, kMetroVariants, dflt, [&](int var) -> int {
     if (!scratch) {
       KLFT_HIP(hipMalloc(&scratch, f->link_bytes));

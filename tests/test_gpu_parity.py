@@ -184,15 +184,15 @@ def test_fp32_plaquette_within_1e6(nc, storage):
     assert abs(got - ref) < 1e-6
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11])
 @pytest.mark.parametrize("nc,beta,storage", [(1, 2.0, "f64"), (2, 2.3, "f64"), (3, 5.5, "f64"),
                                              (3, 5.5, "f32"), (2, 2.3, "f32_rows")])
 def test_metropolis_variants_agree(monkeypatch, variant, nc, beta, storage):
     """Every sweep variant (one lane per site; quad per site at three occupancy
     targets; one lane with the low-register staple and/or proposals drawn in
     pairs; one lane building the next staple during the hits, in registers or
-    staged through LDS; a pair of lanes per site at two occupancy targets)
-    makes the same decisions: FP64 against the oracle, FP32 against
+    staged through LDS; a pair of lanes per site at two occupancy targets; one
+    lane with the site's own links in registers) makes the same decisions: FP64 against the oracle, FP32 against
     variant 0."""
     nd, dims = 4, (4, 6, 4, 4)
     U = orc.random_algebra(nd, nc, dims, 0.3, 19)
