@@ -570,7 +570,7 @@ __global__ __launch_bounds__(256, OCC) void k_staple_field(Links<R, NC, COMP> f,
   int x[ND], mu;
   long long s;
   if (!link_site<ND>(g, lb, x, &s, &mu)) return;
-  o.store(s, mu, staple_lowreg<R, NC, COMP, ND>(f, g, x, mu));
+  o.store(s, mu, staple_lowreg<R, NC, COMP, ND>(f, make_nbr<ND>(g, x, s), mu));
 }
 
 extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {

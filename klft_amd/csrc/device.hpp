@@ -435,6 +435,53 @@ __device__ __forceinline__ long long idx_shift_k(const Geom& g, const int* x, in
   return dev_index<ND>(g, v);
 }
 
+// Device-index offsets of a site's +-1 neighbours.  x +- e_d lies in the
+// sublattice with bit d flipped (+-Vh << (nd-1-d)) and at half-coordinate
+// y_d or y_d +- 1, so every neighbour of x -- and x + e_mu - e_nu -- is s
+// plus one or two of these offsets: additions in place of dev_index's
+// coordinate arithmetic and multiplications on every neighbour load.
+__device__ __forceinline__ long long enc_dim(const Geom& g, int d, int y) {
+  return (long long)(y >> g.tb[d]) * g.ts[d] + ((y & ((1 << g.tb[d]) - 1)) << g.lsh[d]);
+}
+template <int ND>
+struct Nbr {
+  long long s;
+  long long dp[ND], dm[ND];
+  // runtime direction: compare-select chain, no private-array indexing
+  __device__ __forceinline__ long long p(int d) const {
+    long long v = dp[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) v = d == k ? dp[k] : v;
+    return v;
+  }
+  __device__ __forceinline__ long long m(int d) const {
+    long long v = dm[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) v = d == k ? dm[k] : v;
+    return v;
+  }
+  __device__ __forceinline__ long long up(int mu) const { return s + p(mu); }              // x + e_mu
+  __device__ __forceinline__ long long dn(int nu) const { return s + m(nu); }              // x - e_nu
+  __device__ __forceinline__ long long updn(int mu, int nu) const { return s + p(mu) + m(nu); }
+};
+template <int ND>
+__device__ __forceinline__ Nbr<ND> make_nbr(const Geom& g, const int* x, long long s) {
+  Nbr<ND> n;
+  n.s = s;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const long long D = (long long)g.Vh << (ND - 1 - d);
+    const int y = x[d] >> 1;
+    const bool odd = (x[d] & 1) != 0;
+    const long long e0 = enc_dim(g, d, y);
+    const int yp = odd ? (y + 1 == g.H[d] ? 0 : y + 1) : y;
+    const int ym = odd ? y : (y == 0 ? g.H[d] - 1 : y - 1);
+    n.dp[d] = (odd ? -D : D) + (enc_dim(g, d, yp) - e0);
+    n.dm[d] = (odd ? -D : D) + (enc_dim(g, d, ym) - e0);
+  }
+  return n;
+}
+
 // ---------------------------------------------------------------------------
 // link field access
 // ---------------------------------------------------------------------------
@@ -510,6 +557,32 @@ __device__ __forceinline__ Mat<R, NC> staple(const Links<R, NC, COMP>& f, const 
   }
   return t;
 }
+// The same with the neighbours' device indices from precomputed offsets (the
+// one-thread-per-link kernels: less integer work per link; the Metropolis
+// kernels measured faster recomputing indices from the coordinates).
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ Mat<R, NC> staple(const Links<R, NC, COMP>& f, const Nbr<ND>& n, int mu) {
+  Mat<R, NC> t = zero<R, NC>();
+  const long long s_xpmu = n.up(mu);
+#pragma unroll
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == mu) continue;
+    const Mat<R, NC> a = f.load(s_xpmu, nu);
+    const Mat<R, NC> b = f.load(n.up(nu), mu);
+    const Mat<R, NC> c = f.load(n.s, nu);
+    add_to(t, mul_ad(mul_ad(a, b), c));
+  }
+#pragma unroll
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == mu) continue;
+    const long long s_xmnu = n.dn(nu);
+    const Mat<R, NC> a = f.load(n.updn(mu, nu), nu);
+    const Mat<R, NC> b = f.load(s_xmnu, mu);
+    const Mat<R, NC> c = f.load(s_xmnu, nu);
+    add_to(t, mul(mul_dd(a, b), c));
+  }
+  return t;
+}
 
 // The same staple with one term in flight at a time: the compiler cannot
 // hoist all 2(nd-1)*3 link loads, which keeps register use low enough for
@@ -531,6 +604,25 @@ __device__ __forceinline__ Mat<R, NC> staple_lowreg(const Links<R, NC, COMP>& f,
     if (nu == mu) continue;
     const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
     const Mat<R, NC> ab = mul_dd(f.load(idx_shift<ND>(g, x, mu, 1, nu, -1), nu), f.load(s_xmnu, mu));
+    add_to(t, mul(ab, f.load(s_xmnu, nu)));
+  }
+  return t;
+}
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ Mat<R, NC> staple_lowreg(const Links<R, NC, COMP>& f, const Nbr<ND>& n, int mu) {
+  Mat<R, NC> t = zero<R, NC>();
+  const long long s_xpmu = n.up(mu);
+#pragma unroll 1
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == mu) continue;
+    const Mat<R, NC> ab = mul_ad(f.load(s_xpmu, nu), f.load(n.up(nu), mu));
+    add_to(t, mul_ad(ab, f.load(n.s, nu)));
+  }
+#pragma unroll 1
+  for (int nu = 0; nu < ND; ++nu) {
+    if (nu == mu) continue;
+    const long long s_xmnu = n.dn(nu);
+    const Mat<R, NC> ab = mul_dd(f.load(n.updn(mu, nu), nu), f.load(s_xmnu, mu));
     add_to(t, mul(ab, f.load(s_xmnu, nu)));
   }
   return t;

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, 
   long long s;
   double k = 0.0;
   if (link_site<ND>(g, lb, x, &s, &mu)) {
-    const Mat<R, NC> S = staple_lowreg<R, NC, COMP, ND>(f, g, x, mu);
+    const Mat<R, NC> S = staple_lowreg<R, NC, COMP, ND>(f, make_nbr<ND>(g, x, s), mu);
     const Mat<R, NC> M = mul(f.load(s, mu), S);
     R F[NG];
     project(M, coef, F);
