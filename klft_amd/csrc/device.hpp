@@ -3,17 +3,17 @@
 // navigation, Metropolis proposals, reunitarisation, the Lie-algebra
 // exponential and block / grid reductions.
 //
-// Link layout in HBM (one allocation per field):
-//   links[(mu * NCOMP + c) * V + s]  with  s = sub * Vh + r
+// Link layout in HBM (one allocation per field), blocks of 64 sites:
+//   element (s, mu, c) at ((s / 64) * nd + mu) * NCOMP * 64 + c * 64 + s % 64
+//   s   = sub * Vh + r  (device index)
 //   sub = sum_d (x_d & 1) << (nd - 1 - d)   -- the sublattice (pass) index of
 //         the reference's oddeven_array (IndexHelper.hpp:84-93)
-//   r   = lexicographic index of y_d = x_d >> 1 over the half extents, last
-//         dimension fastest (index_odd_even, IndexHelper.hpp:68-81)
+//   r   = position of y_d = x_d >> 1 inside the sublattice (Geom below)
 //   c   = row-major colour component; NCOMP = Nc*Nc, or (Nc-1)*Nc for the
 //         compressed two-row (SU(3)) / one-row (SU(2)) storage.
 // Each Metropolis pass therefore reads and writes whole contiguous
-// (sublattice, direction) groups, and every load is a 64-lane coalesced
-// 8- or 16-byte access.
+// (sublattice, direction) groups, and a wave's load of one link is one
+// contiguous run of NCOMP * 64 complex numbers.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -311,11 +311,11 @@ __device__ __forceinline__ void reunitarize(Mat<float, N>& a) {
 // lattice geometry
 // ---------------------------------------------------------------------------
 // Within each sublattice the half-extent coordinates y_d = x_d >> 1 are
-// ordered in 4D tiles of power-of-two extent (4 where H_d = L_d/2 allows it):
+// ordered in 4D tiles of power-of-two extent T_d = 2^tb_d:
 //   r = sum_d (y_d >> tb_d) ts_d + sum_d (y_d & (T_d - 1)) << lsh_d,
 // tiles in lexicographic order (last dimension fastest), sites inside a tile
-// likewise.  A tile's +-1 neighbours then lie in the same tile of another
-// sublattice, which the launch order keeps resident in L2.
+// likewise.  The library uses T_d = 1 (plain lexicographic order) unless
+// KLFT_TILES=1 asks for T_d = 4 where H_d = L_d/2 allows it.
 struct Geom {
   int nd;
   int L[4];

@@ -17,7 +17,7 @@ struct klft_field_s {
   int nd = 0, nc = 0, storage = 0;
   int dims[4] = {1, 1, 1, 1};
   kd::Geom g{};
-  void *links = nullptr;  // SoA, sublattice-major (device.hpp)
+  void *links = nullptr;  // blocks of 64 sites, sublattice-major (device.hpp)
   size_t link_bytes = 0;
   int ncomp = 0;          // complex components stored per link
   size_t cbytes = 0;      // bytes per complex component
@@ -146,21 +146,21 @@ inline unsigned int blocks_for(long long n, int block) {
   return (unsigned int)((n + block - 1) / block);
 }
 
-// workgroups are dispatched round-robin over the 8 XCDs; this gives each XCD
-// a contiguous range of logical block indices so neighbouring tiles share an
-// L2.
 }  // namespace klft_internal
 
 namespace kd {
+// Workgroups are dispatched round-robin over the 8 XCDs; this gives each XCD
+// a contiguous range of logical block indices, so blocks working on
+// neighbouring sites share an L2.
 __device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int nb) {
   const unsigned int per = nb / 8u, rem = nb % 8u;
   const unsigned int x = b % 8u, k = b / 8u;
   return (x < rem ? x * (per + 1u) : rem * (per + 1u) + (x - rem) * per) + k;
 }
 // One thread per site: logical block lb covers 256 consecutive positions r
-// (one 4^4 tile when the extents allow it) of sublattice lb % 2^nd, so 2^nd
-// consecutive logical blocks -- kept on one XCD by xcd_block -- cover a tile
-// of every sublattice while each wave reads 64 consecutive sites.
+// of sublattice lb % 2^nd, so 2^nd consecutive logical blocks -- kept on one
+// XCD by xcd_block -- cover the same region of every sublattice while each
+// wave reads 64 consecutive sites.
 template <int ND>
 __device__ __forceinline__ bool chunk_site(const Geom &g, unsigned int lb, int *x, long long *s) {
   constexpr int NSUB = 1 << ND;
