@@ -31,7 +31,8 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-f
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INCLUDE}",
           "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
 
-HIP_SOURCES = ["core.hip", "metropolis.hip", "hmc.hip", "tuner.hip", "batch.hip"]
+HIP_SOURCES = ["core.hip", "metropolis.hip", "metropolis_f64.hip", "metropolis_f32.hip", "metropolis_f32rows.hip",
+               "hmc.hip", "tuner.hip", "batch.hip"]
 HOST_SOURCES = ["host/driver.cpp"]
 TOOLS = {"metropolis": "tools/metropolis_main.cpp", "hmc": "tools/hmc_main.cpp",
          "plaq_bench": "tools/plaq_bench.cpp", "wloop_temporal_bench": "tools/wloop_bench.cpp",

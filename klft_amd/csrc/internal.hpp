@@ -126,6 +126,28 @@ int dispatch(int storage, int nc, int nd, F &&fn) {
   return set_error(KLFT_E_ARG, "storage=%d unsupported", storage);
 }
 
+// the same for one storage type (R, compressed rows) fixed at compile time;
+// U(1) has no compressed form, so COMP = true covers Nc = 2, 3 only
+template <typename RR, bool CC, class F>
+int dispatch_typed(int nc, int nd, F &&fn) {
+  auto by_nd = [&](auto nct, auto ct) -> int {
+    switch (nd) {
+      case 2: return fn(tag<RR>{}, nct, ct, ic<2>{});
+      case 3: return fn(tag<RR>{}, nct, ct, ic<3>{});
+      case 4: return fn(tag<RR>{}, nct, ct, ic<4>{});
+    }
+    return set_error(KLFT_E_ARG, "nd=%d unsupported", nd);
+  };
+  switch (nc) {
+    case 1:
+      if constexpr (!CC) return by_nd(ic<1>{}, ic<0>{});
+      break;
+    case 2: return by_nd(ic<2>{}, ic<CC ? 1 : 0>{});
+    case 3: return by_nd(ic<3>{}, ic<CC ? 1 : 0>{});
+  }
+  return set_error(KLFT_E_ARG, "Nc=%d unsupported for this storage", nc);
+}
+
 #define KLFT_UNPACK(rt, nct, ct, ndt)              \
   using R = typename decltype(rt)::type;           \
   constexpr int NC = decltype(nct)::value;         \
