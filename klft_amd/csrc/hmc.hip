@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_refresh(R *P, Geom g, uint64_t seed, ui
 // P -= dtau * F with F recomputed from the current links.  Fout (optional):
 // write F in host AoS order instead of updating P.  kin (optional): also
 // reduce 1/2 sum p^2 of the updated momenta.
-template <typename R, int NC, bool COMP, int ND, int OCC>
+template <typename R, int NC, bool COMP, int ND, int OCC, bool LOWREG = true>
 __global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, Geom g,
                                                                      R coef, R dtau, double *Fout,
                                                                      double *partials,
@@ -81,7 +81,12 @@ __global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, 
   long long s;
   double k = 0.0;
   if (link_site<ND>(g, lb, x, &s, &mu)) {
-    const Mat<R, NC> S = staple_lowreg<R, NC, COMP, ND>(f, make_nbr<ND>(g, x, s), mu);
+    const Nbr<ND> nb = make_nbr<ND>(g, x, s);
+    Mat<R, NC> S;
+    if constexpr (LOWREG)
+      S = staple_lowreg<R, NC, COMP, ND>(f, nb, mu);
+    else
+      S = staple<R, NC, COMP, ND>(f, nb, mu);
     const Mat<R, NC> M = mul(f.load(s, mu), S);
     R F[NG];
     project(M, coef, F);
@@ -99,6 +104,52 @@ __global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, 
         n2 += (double)v * (double)v;
       }
       k = 0.5 * n2;
+    }
+  }
+  if (kin) {
+    const double bt = block_sum<256>(k, lds);
+    grid_finish<256>(bt, partials, ticket, kin, lds);
+  }
+}
+
+// The same update with one thread per site for all its directions, one
+// after the other.  The 76 staple-link loads of a site then come from one
+// wave within a short time, so the links its directions share (each pair
+// U_nu(x+mu), U_mu(x+nu) twice, U_nu(x-nu) three times, the site's own links
+// four times: 44 distinct) are re-read from the caches instead of from
+// memory, where the one-thread-per-link kernel spreads them over four waves.
+template <typename R, int NC, bool COMP, int ND>
+__global__ __launch_bounds__(256) void k_force_site(Links<R, NC, COMP> f, R *P, Geom g, R coef, R dtau,
+                                                    double *Fout, double *partials, unsigned int *ticket,
+                                                    double *kin) {
+  __shared__ double lds[8];
+  constexpr int NG = Ngen<NC>::value;
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  int x[ND];
+  long long s;
+  double k = 0.0;
+  if (chunk_site<ND>(g, lb, x, &s)) {
+    const Nbr<ND> nb = make_nbr<ND>(g, x, s);
+#pragma unroll 1
+    for (int mu = 0; mu < ND; ++mu) {
+      const Mat<R, NC> S = staple_lowreg<R, NC, COMP, ND>(f, nb, mu);
+      R F[NG];
+      project(mul(f.load(s, mu), S), coef, F);
+      if (Fout) {
+        const long long site = lex_index<ND>(g, x);
+#pragma unroll
+        for (int a = 0; a < NG; ++a) Fout[(site * ND + mu) * NG + a] = (double)F[a];
+      } else {
+        double n2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < NG; ++a) {
+          R *q = P + ((long long)mu * NG + a) * g.V + s;
+          const R v = *q - dtau * F[a];
+          *q = v;
+          n2 += (double)v * (double)v;
+        }
+        k += 0.5 * n2;
+      }
     }
   }
   if (kin) {
@@ -190,12 +241,35 @@ int enqueue_refresh(klft_hmc_t h, uint64_t traj, int slot) {
   });
 }
 
+// variants: 0, 1, 2 one thread per link with >= 1, 2, 4 waves/SIMD; 3 one
+// thread per site (k_force_site); 4 one thread per link with all staple
+// loads in flight at once (full staple)
+constexpr int kForceVariants = 5;
+
 int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int variant) {
   klft_field_t f = h->f;
   return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
+    if (variant == 3) {
+      const unsigned nbs = ki::blocks_for(chunk_threads<ND>(f->g), 256);
+      if (nbs > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+      ki::ProfScope prof("force");
+      hipLaunchKernelGGL((k_force_site<R, NC, COMP, ND>), dim3(nbs), dim3(256), 0, ki::stream(),
+                         ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC), (R)dtau,
+                         Fout, f->partials, f->ticket, kin_slot >= 0 ? f->result + kin_slot : (double *)nullptr);
+      KLFT_HIP(hipGetLastError());
+      return KLFT_OK;
+    }
     const unsigned nb = (unsigned)link_blocks<ND>(f->g);
     if (nb > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+    if (variant == 4) {
+      ki::ProfScope prof("force");
+      hipLaunchKernelGGL((k_force<R, NC, COMP, ND, 1, false>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
+                         ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC), (R)dtau,
+                         Fout, f->partials, f->ticket, kin_slot >= 0 ? f->result + kin_slot : (double *)nullptr);
+      KLFT_HIP(hipGetLastError());
+      return KLFT_OK;
+    }
     return ki::with_occ(variant, [&](auto occ) -> int {
       constexpr int OCC = decltype(occ)::value;
       ki::ProfScope prof("force");
@@ -211,7 +285,9 @@ int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int vari
 
 // the tuning launcher uses dtau = 0, which leaves the momenta unchanged
 int enqueue_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot) {
-  const int v = ki::pick_variant("force", h->f, ki::kOccVariants, 0,
+  // untuned default, the fastest at 32^4: the full staple for FP64 links
+  const int dflt = h->f->storage == KLFT_STORAGE_F64 ? 4 : 0;
+  const int v = ki::pick_variant("force", h->f, kForceVariants, dflt,
                                  [&](int var) { return launch_force(h, 0.0, nullptr, -1, var); });
   return launch_force(h, dtau, Fout, kin_slot, v);
 }

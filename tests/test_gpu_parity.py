@@ -210,3 +210,28 @@ def test_metropolis_variants_agree(monkeypatch, variant, nc, beta, storage):
         g0 = klft.GaugeField(nd, nc, dims, storage).upload(U)
         n_0 = [klft.sweep_Metropolis(g0, p, s, with_count=True)[1] for s in range(2)]
         assert abs(n_g[0] - n_0[0]) <= 2 and abs(n_g[1] - n_0[1]) <= 4
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("nd,nc,dims", [(4, 3, (4, 4, 6, 4)), (3, 2, (4, 6, 4)), (4, 1, (6, 4, 4, 4))])
+def test_measurement_and_force_variants_match_oracle(monkeypatch, variant, nd, nc, dims):
+    """Every plaquette / staple-field variant (occupancy 1, 2, 4) and every
+    force variant (one thread per link at three occupancies, one thread per
+    site, one thread per link with the full staple) against the oracle."""
+    U = seeded(nd, nc, dims, 0.6, 41)
+    monkeypatch.setenv("KLFT_VARIANT", f"plaquette={min(variant, 2)},staple_field={min(variant, 2)},force={variant}")
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    got, ps = klft.GaugePlaquette(g, per_site=True)
+    want, ps_o = orc.plaquette(U, nd, nc, dims, per_site=True)
+    np.testing.assert_allclose(ps, ps_o, rtol=0, atol=1e-12)
+    assert abs(got - want) < 1e-12
+    np.testing.assert_allclose(klft.stapleField(g).download(), orc.staple_field(U, nd, nc, dims), rtol=0, atol=1e-12)
+    h = klft.HMC(g, beta=2.5, tau=1.0, nsteps=4, seed=9)
+    np.testing.assert_allclose(h.force(), orc.force(U, nd, nc, dims, 2.5), rtol=0, atol=1e-12)
+    # the momentum update itself (P -= dtau F) through the same variant
+    h.refresh_momenta(1)
+    P0 = h.momenta()
+    h.update_momentum(0.125)
+    P1 = P0.copy()
+    orc.update_momentum(P1, orc.force(U, nd, nc, dims, 2.5), nd, nc, dims, 0.125)
+    np.testing.assert_allclose(h.momenta(), P1, rtol=0, atol=1e-12)
