@@ -563,14 +563,20 @@ extern "C" int klft_plaquette(klft_field_t f, int normalize, double *out, double
 // ============================================================================
 // staple field (Gauge_Util.hpp:27-97)
 // ============================================================================
-template <typename R, int NC, bool COMP, int ND, int OCC>
+// variants 0, 1, 2: one-term-at-a-time staple at >= 1, 2, 4 waves/SIMD;
+// 3: full staple (all its loads in flight at once)
+template <typename R, int NC, bool COMP, int ND, int OCC, bool LOWREG = true>
 __global__ __launch_bounds__(256, OCC) void k_staple_field(Links<R, NC, COMP> f,
                                                                             Links<R, NC, false> o, Geom g) {
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   int x[ND], mu;
   long long s;
   if (!link_site<ND>(g, lb, x, &s, &mu)) return;
-  o.store(s, mu, staple_lowreg<R, NC, COMP, ND>(f, make_nbr<ND>(g, x, s), mu));
+  const Nbr<ND> nb = make_nbr<ND>(g, x, s);
+  if constexpr (LOWREG)
+    o.store(s, mu, staple_lowreg<R, NC, COMP, ND>(f, nb, mu));
+  else
+    o.store(s, mu, staple<R, NC, COMP, ND>(f, nb, mu));
 }
 
 extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
@@ -586,6 +592,13 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
     return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
       KLFT_UNPACK(rt, nct, ct, ndt);
       const unsigned nb = (unsigned)link_blocks<ND>(f->g);
+      if (variant == 3) {
+        ki::ProfScope prof("staple_field");
+        hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND, 1, false>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
+                           ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
+        KLFT_HIP(hipGetLastError());
+        return KLFT_OK;
+      }
       return ki::with_occ(variant, [&](auto occ) -> int {
         constexpr int OCC = decltype(occ)::value;
         ki::ProfScope prof("staple_field");
@@ -596,7 +609,7 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
       });
     });
   };
-  rc = launch(ki::pick_variant("staple_field", f, ki::kOccVariants, 0, launch));
+  rc = launch(ki::pick_variant("staple_field", f, ki::kOccVariants + 1, f->storage == KLFT_STORAGE_F64 ? 3 : 0, launch));
   if (rc) return rc;
   return ki::sync_stream();
 }
