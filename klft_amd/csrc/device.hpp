@@ -647,6 +647,25 @@ __device__ __forceinline__ double plaq_at(const Links<R, NC, COMP>& f, const Geo
   return p;
 }
 
+// the same sum with the site's own links loaded once and every plane's
+// loads in flight together (same term order, so the same result)
+template <typename R, int NC, bool COMP, int ND>
+__device__ __forceinline__ double plaq_at_full(const Links<R, NC, COMP>& f, const Nbr<ND>& n) {
+  Mat<R, NC> U[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) U[d] = f.load(n.s, d);
+  double p = 0.0;
+#pragma unroll
+  for (int mu = 0; mu < ND - 1; ++mu)
+#pragma unroll
+    for (int nu = mu + 1; nu < ND; ++nu) {
+      const Mat<R, NC> lmu = mul(U[mu], f.load(n.s + n.dp[mu], nu));
+      const Mat<R, NC> lnu = mul(U[nu], f.load(n.s + n.dp[nu], mu));
+      p += (double)retr_mul_ad(lmu, lnu);
+    }
+  return p;
+}
+
 // waves per SIMD requested from the register allocator
 template <typename R, int NC>
 struct Occ {
