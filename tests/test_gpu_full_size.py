@@ -1,0 +1,73 @@
+"""GPU parity at the benchmark's own size: SU(3) on 32^4 (and 8^4).
+
+The oracle's FP64 kernels run these sizes in seconds on the host cores, so
+the full-size checks compare with it directly: the seeded start field, the
+per-site plaquette, one complete Metropolis sweep (nHits = 10, beta = 6.0,
+delta = 0.1, the headline workload), the HMC force, and FP32 / FP32 two-row
+plaquettes within 1e-6 of FP64.  Size-independent properties ride along:
+links stay unitary to rounding, and the cold field's plaquette is exactly 1.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+klft = pytest.importorskip("klft_amd")
+
+
+@pytest.fixture(scope="module", params=[8, 32])
+def field(request):
+    L = request.param
+    dims = (L, L, L, L)
+    U = orc.random_algebra(4, 3, dims, 0.5, 2024)
+    return L, dims, U
+
+
+def test_seeded_start_and_plaquette(field):
+    L, dims, U = field
+    g = klft.GaugeField(4, 3, dims, "f64").set_random_algebra(0.5, 2024)
+    np.testing.assert_allclose(g.download(), U, rtol=0, atol=1e-14)
+    got, ps = klft.GaugePlaquette(g, per_site=True)
+    want, ps_o = orc.plaquette(U, 4, 3, dims, per_site=True)
+    np.testing.assert_allclose(ps, ps_o, rtol=0, atol=1e-12)
+    assert abs(got - want) <= 1e-12 * np.abs(ps_o).sum() / (np.prod(dims) * 6 * 3) + 1e-15
+
+
+def test_cold_field_is_exactly_one(field):
+    L, dims, _ = field
+    g = klft.GaugeField(4, 3, dims, "f64").set_identity()
+    got, ps = klft.GaugePlaquette(g, per_site=True)
+    _, ps_o = orc.plaquette(orc.identity(4, 3, dims), 4, 3, dims, per_site=True)
+    assert got == 1.0 and np.array_equal(ps, ps_o)
+
+
+def test_full_sweep_matches_oracle(field):
+    L, dims, U = field
+    g = klft.GaugeField(4, 3, dims, "f64").upload(U)
+    p = klft.MetropolisParams(nHits=10, beta=6.0, delta=0.1, seed=4242)
+    acc, n = klft.sweep_Metropolis(g, p, 0, with_count=True)
+    Uo = U.copy()
+    acc_o, n_o = orc.metropolis_sweep(Uo, 4, 3, dims, 10, 6.0, 0.1, 4242, 0)
+    assert n == n_o and abs(acc - acc_o) < 1e-15
+    Ug = g.download()
+    np.testing.assert_allclose(Ug, Uo, rtol=0, atol=1e-11)
+    # unitarity to rounding after the sweep's restoreSUN
+    V = Ug.reshape(-1, 3, 3)
+    dev = np.abs(V @ np.conj(np.transpose(V, (0, 2, 1))) - np.eye(3)).max()
+    assert dev < 1e-13
+
+
+def test_force_matches_oracle(field):
+    L, dims, U = field
+    g = klft.GaugeField(4, 3, dims, "f64").upload(U)
+    h = klft.HMC(g, beta=6.0, tau=1.0, nsteps=20, seed=5)
+    np.testing.assert_allclose(h.force(), orc.force(U, 4, 3, dims, 6.0), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32_rows"])
+def test_fp32_plaquette_within_1e6(field, storage):
+    L, dims, U = field
+    g = klft.GaugeField(4, 3, dims, storage).set_random_algebra(0.5, 2024).reunitarize()
+    assert abs(klft.GaugePlaquette(g) - orc.plaquette(U, 4, 3, dims)) < 1e-6
