@@ -16,7 +16,6 @@ using namespace kd;
 using namespace klft_internal;
 
 constexpr int kMetroBlock = 256;
-constexpr int kMetroVariants = 12;
 
 // LOWREG takes the staple one term at a time; HB proposals are drawn ahead of
 // their Markov steps; OCC asks the register allocator for at least OCC waves
