@@ -310,7 +310,9 @@ extern "C" int klft_field_create(int nd, int nc, const int *dims, int storage, k
     delete f;
     return ki::set_error(KLFT_E_NOMEM, "hipMalloc of %zu link bytes failed", (size_t)V * nd * 16);
   }
-  size_t ws = sizeof(double) * ki::kMaxPartials + 256 + 16 * sizeof(double) + 8 * sizeof(unsigned long long);
+  f->max_partials = std::max<long long>(ki::kMinPartials, kd::link_blocks<4>(f->g) + 64);
+  const size_t pbytes = ((sizeof(double) * (size_t)f->max_partials + 255) / 256) * 256;
+  size_t ws = pbytes + 256 + 16 * sizeof(double) + 8 * sizeof(unsigned long long);
   char *w = nullptr;
   e = hipMalloc((void **)&w, ws);
   if (e != hipSuccess) {
@@ -319,9 +321,9 @@ extern "C" int klft_field_create(int nd, int nc, const int *dims, int storage, k
     return ki::set_error(KLFT_E_NOMEM, "workspace allocation failed");
   }
   f->partials = (double *)w;
-  f->ticket = (unsigned int *)(w + sizeof(double) * ki::kMaxPartials);
-  f->result = (double *)(w + sizeof(double) * ki::kMaxPartials + 256);
-  f->counter = (unsigned long long *)(w + sizeof(double) * ki::kMaxPartials + 256 + 16 * sizeof(double));
+  f->ticket = (unsigned int *)(w + pbytes);
+  f->result = (double *)(w + pbytes + 256);
+  f->counter = (unsigned long long *)(w + pbytes + 256 + 16 * sizeof(double));
   e = hipMemset(w, 0, ws);
   if (e == hipSuccess) e = hipHostMalloc((void **)&f->host_result, 16 * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) {
@@ -514,7 +516,7 @@ static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int 
   return dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
     const unsigned nb = blocks_for(chunk_threads<ND>(f->g), 256);
-    if (nb > (unsigned)kMaxPartials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
+    if ((long long)nb > f->max_partials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
     if (variant == 3) {
       ProfScope prof("plaquette");
       hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, 1, true>), dim3(nb), dim3(256), 0, stream(),

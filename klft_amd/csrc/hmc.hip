@@ -233,7 +233,7 @@ int enqueue_refresh(klft_hmc_t h, uint64_t traj, int slot) {
     KLFT_UNPACK(rt, nct, ct, ndt);
     (void)COMP;
     const unsigned nb = ki::blocks_for(f->g.V, 256);
-    if (nb > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+    if ((long long)nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
     hipLaunchKernelGGL((k_refresh<R, NC, ND>), dim3(nb), dim3(256), 0, ki::stream(), (R *)h->P, f->g,
                        h->p.seed, (uint32_t)traj, f->partials, f->ticket, f->result + slot);
     KLFT_HIP(hipGetLastError());
@@ -252,7 +252,7 @@ int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int vari
     KLFT_UNPACK(rt, nct, ct, ndt);
     if (variant == 3) {
       const unsigned nbs = ki::blocks_for(chunk_threads<ND>(f->g), 256);
-      if (nbs > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+      if ((long long)nbs > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
       ki::ProfScope prof("force");
       hipLaunchKernelGGL((k_force_site<R, NC, COMP, ND>), dim3(nbs), dim3(256), 0, ki::stream(),
                          ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC), (R)dtau,
@@ -261,7 +261,7 @@ int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int vari
       return KLFT_OK;
     }
     const unsigned nb = (unsigned)link_blocks<ND>(f->g);
-    if (nb > (unsigned)ki::kMaxPartials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+    if ((long long)nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
     if (variant == 4) {
       ki::ProfScope prof("force");
       hipLaunchKernelGGL((k_force<R, NC, COMP, ND, 1, false>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
@@ -310,7 +310,7 @@ int enqueue_kinetic(klft_hmc_t h, int slot) {
     KLFT_UNPACK(rt, nct, ct, ndt);
     (void)COMP;
     (void)ND;
-    if (ki::blocks_for(f->g.V, 256) > (unsigned)ki::kMaxPartials)
+    if ((long long)ki::blocks_for(f->g.V, 256) > f->max_partials)
       return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
     hipLaunchKernelGGL((k_kinetic_soa<R, Ngen<NC>::value>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256),
                        0, ki::stream(), (const R *)h->P, f->g.V, f->nd, f->partials, f->ticket,

@@ -23,6 +23,7 @@ struct klft_field_s {
   size_t cbytes = 0;      // bytes per complex component
   // reduction workspace: per-block partials, a ticket and result slots
   double *partials = nullptr;
+  long long max_partials = 0;  // capacity of partials: the largest reduction grid
   unsigned int *ticket = nullptr;
   double *result = nullptr;  // 16 doubles
   unsigned long long *counter = nullptr;  // 8 counters
@@ -31,7 +32,9 @@ struct klft_field_s {
 
 namespace klft_internal {
 
-constexpr int kMaxPartials = 1 << 16;
+// reduction partials per field: at least this many, and at least one per
+// block of the largest grid (one thread per link, 64 sites per block)
+constexpr long long kMinPartials = 1 << 16;
 
 int set_error(int code, const char *fmt, ...);
 int ensure_init();

@@ -71,3 +71,29 @@ def test_fp32_plaquette_within_1e6(field, storage):
     L, dims, U = field
     g = klft.GaugeField(4, 3, dims, storage).set_random_algebra(0.5, 2024).reunitarize()
     assert abs(klft.GaugePlaquette(g) - orc.plaquette(U, 4, 3, dims)) < 1e-6
+
+
+def test_lattice_beyond_65536_reduction_blocks():
+    """48^3 x 40 (4.4 M sites, 69,120 link blocks): every reduction grid is
+    larger than 2^16 blocks.  The field is an 8^4 oracle field repeated
+    periodically, so the plaquette, per-site plaquette, force and kinetic
+    energy are the 8^4 oracle values repeated (sums scaled by the 1080 copies).
+    """
+    small = (8, 8, 8, 8)
+    reps = (6, 6, 6, 5)
+    dims = tuple(a * b for a, b in zip(small, reps))
+    U8 = orc.random_algebra(4, 3, small, 0.5, 77)
+    want_plaq, ps8 = orc.plaquette(U8, 4, 3, small, per_site=True)
+    F8 = orc.force(U8, 4, 3, small, 6.0)
+    g = klft.GaugeField(4, 3, dims, "f64").upload(np.tile(U8, reps + (1, 1, 1)))
+    got, ps = klft.GaugePlaquette(g, per_site=True)
+    assert abs(got - want_plaq) < 1e-12
+    np.testing.assert_allclose(ps.reshape(dims), np.tile(ps8.reshape(small), reps), rtol=0, atol=1e-12)
+    del ps
+    h = klft.HMC(g, beta=6.0, tau=1.0, nsteps=20, seed=5)
+    np.testing.assert_allclose(h.force(), np.tile(F8, reps + (1, 1)), rtol=0, atol=1e-12)
+    P8 = np.random.default_rng(3).standard_normal(small + (4, 8))
+    h.set_momenta(np.tile(P8, reps + (1, 1)))
+    h8 = klft.HMC(klft.GaugeField(4, 3, small, "f64").upload(U8), beta=6.0, tau=1.0, nsteps=20, seed=5)
+    h8.set_momenta(P8)
+    assert abs(h.kinetic() - 1080 * h8.kinetic()) <= 1e-12 * abs(h.kinetic())
