@@ -102,6 +102,21 @@ __device__ __forceinline__ Mat<R, NC> staple_c(const Links<R, NC, COMP> &f, cons
 // sequence restoreSUN(U r1), restoreSUN(U r2), ... but a wave pays for that
 // work once per link instead of at every hit where any of its 64 lanes
 // accepts (about half the hits at 1 % acceptance).
+// accept iff dS < 0 or u < e^{-dS} (Metropolis.hpp:96-105)
+template <typename R>
+__device__ __forceinline__ bool metro_accept(R dS, R u) {
+  bool accept = dS < R(0);
+  if (!accept) {
+    // e^x > 1 + x + x^2/2 + x^3/6 for x > 0, so u * that cubic >= 1 means
+    // u > e^{-dS}: a rejection found without the exponential.  dS > 1/2
+    // keeps the margin (> 0.1 %) far above rounding, so the decision is
+    // always the one u < e^{-dS} gives.
+    const R p3 = R(1) + dS * (R(1) + dS * (R(0.5) + dS * (R(1) / R(6))));
+    if (!(dS > R(0.5) && u * p3 >= R(1))) accept = u < exp(-dS);
+  }
+  return accept;
+}
+
 template <typename R, int NC>
 struct LinkChain {
   Mat<R, NC> U, M, P;
@@ -122,17 +137,7 @@ struct LinkChain {
     }
   }
   __device__ __forceinline__ void step(const Mat<R, NC> &r, R u, R beta_nc, R cl, unsigned long long &n) {
-    const R dS = cl * (-beta_nc * (retr_mul(r, M) - trM));
-    bool accept = dS < R(0);
-    if (!accept) {
-      // e^x > 1 + x + x^2/2 + x^3/6 for x > 0, so u * that cubic >= 1 means
-      // u > e^{-dS}: a rejection found without the exponential.  dS > 1/2
-      // keeps the margin (> 0.1 %) far above rounding, so the decision is
-      // always the one u < e^{-dS} gives.
-      const R p3 = R(1) + dS * (R(1) + dS * (R(0.5) + dS * (R(1) / R(6))));
-      if (!(dS > R(0.5) && u * p3 >= R(1))) accept = u < exp(-dS);
-    }
-    if (accept) {
+    if (metro_accept(cl * (-beta_nc * (retr_mul(r, M) - trM)), u)) {
       apply_pending();
       P = r;
       pending = true;

@@ -113,6 +113,211 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe_lds(
 }
 
 // ---------------------------------------------------------------------------
+// Site waves and staple waves.  At 32^4 a pass has one wave of sites per
+// SIMD, and the pipelined kernel's single wave cannot keep the next
+// direction's staple loads in flight while it computes hits: its pass costs
+// about the staple phase plus the hit phase.  Here a block of 8 waves holds
+// 4 site waves and 4 staple waves, two waves per SIMD.  Site wave i runs the
+// Markov chains of its 64 sites; staple wave i + 4 builds the staple of the
+// next direction for the same sites and hands it over through LDS, so the
+// SIMD computes hits while the other wave waits on memory.  The staple is the
+// pipelined kernel's (same terms, same order): every term except the upper
+// one through U_mu(x), which the site wave adds once U_mu(x) is final, from
+// U_mu(x+m) U_m(x+mu)^dagger left by the staple wave.  The site's own link
+// is handed over the same way.  Per direction the block meets two barriers:
+// after the site waves have read the handover buffers, and after the staple
+// waves have refilled them and the site waves have stored the updated link
+// (which the next staple reads as U_nu(x), nu < m).  Every wave of the block
+// runs the same loop, so every barrier is met by all eight.
+// ---------------------------------------------------------------------------
+constexpr int kWsBlock = 512;
+constexpr int kWsSites = 256;
+
+template <typename R, int NC>
+__device__ __forceinline__ void lds_put(cx<R> *buf, const Mat<R, NC> &a) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < NC * NC; ++c) buf[c * 64 + l] = a.m[c / NC][c % NC];
+}
+template <typename R, int NC>
+__device__ __forceinline__ Mat<R, NC> lds_get(const cx<R> *buf) {
+  const int l = threadIdx.x & 63;
+  Mat<R, NC> a;
+#pragma unroll
+  for (int c = 0; c < NC * NC; ++c) a.m[c / NC][c % NC] = buf[c * 64 + l];
+  return a;
+}
+
+// LinkChain with U in the site wave's LDS buffer `bu`: U is read only when
+// an accepted proposal is multiplied in, and two waves per SIMD leave each
+// wave half the registers of the one-wave kernels.
+template <typename R, int NC>
+struct LdsChain {
+  Mat<R, NC> M, P;
+  R trM;
+  bool pending;
+  cx<R> *bu;
+
+  __device__ __forceinline__ void start(const Mat<R, NC> &S, const Mat<R, NC> &U0) {
+    lds_put<R, NC>(bu, U0);
+    M = mul(S, U0);
+    trM = retrace(M);
+    pending = false;
+  }
+  __device__ __forceinline__ void apply_pending() {
+    if (pending) {
+      Mat<R, NC> U = mul(lds_get<R, NC>(bu), P);
+      reunitarize(U);
+      lds_put<R, NC>(bu, U);
+      pending = false;
+    }
+  }
+  __device__ __forceinline__ void step(const Mat<R, NC> &r, R u, R beta_nc, unsigned long long &n) {
+    if (metro_accept(-beta_nc * (retr_mul(r, M) - trM), u)) {
+      apply_pending();
+      P = r;
+      pending = true;
+      M = mul(M, r);
+      trM = retrace(M);
+      ++n;
+    }
+  }
+};
+
+template <typename R, int NC>
+__device__ __forceinline__ Mat<R, NC> adjoint(const Mat<R, NC> &a) {
+  Mat<R, NC> c;
+#pragma unroll
+  for (int i = 0; i < NC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) c.m[i][j] = mk(a.m[j][i].re, -a.m[j][i].im);
+  return c;
+}
+
+// ILV: each nu's upper and lower staple terms back to back (else all upper
+// terms first, the order of the other kernels)
+template <typename R, int NC, bool COMP, int ND, bool ILV>
+__global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
+    Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
+    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  constexpr int NM = NC * NC;
+  // per site wave: the staple (all terms but the deferred one), the deferred
+  // term's first product, the own link of the next direction, and the link
+  // being updated
+  __shared__ cx<R> stage[4][4][NM * 64];
+  __shared__ unsigned long long red[kWsBlock / 64];
+  const int w = (int)(threadIdx.x >> 6);
+  const int wi = w & 3;
+  const bool sitew = w < 4;
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long t = (long long)lb * kWsSites + wi * 64 + (threadIdx.x & 63);
+  const bool active = t < g.Vh;
+  int x[ND];
+  long long s = 0;
+  uint32_t site = 0;
+  if (active) {
+    sub_coords<ND>(g, pass, t, x);
+    s = (long long)pass * g.Vh + t;
+    site = (uint32_t)lex_index<ND>(g, x);
+  }
+  cx<R> *bS = stage[wi][0], *bA = stage[wi][1], *bU = stage[wi][2];
+  unsigned long long n = 0;
+  // the two roles run in separate branches, so neither role's registers are
+  // held across the other's code; each executes 1 + 2 nd barriers
+  if (sitew) {
+    if (active && visit) visit[site] = pass;
+    LdsChain<R, NC> lc;
+    lc.bu = stage[wi][3];
+    __syncthreads();
+#pragma unroll 1
+    for (int mu = 0; mu < ND; ++mu) {
+      if (active) {
+        Mat<R, NC> S = lds_get<R, NC>(bS);
+        if (mu > 0) add_to(S, mul_ad(lds_get<R, NC>(bA), lds_get<R, NC>(lc.bu)));  // the final U_{mu-1}(x)
+        lc.start(S, lds_get<R, NC>(bU));
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll 1
+        for (int hit = 0; hit < nhits; ++hit) {
+          Mat<R, NC> r;
+          R u;
+          propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
+          lc.step(r, u, beta_nc, n);
+        }
+        lc.apply_pending();
+        if (nhits > 0) f.store(s, mu, lds_get<R, NC>(lc.bu));
+      }
+      __syncthreads();
+    }
+  } else {
+    // staple waves.  keep carries U_{sg+1}(x+sg) U_sg(x+sg+1)^dagger, the
+    // first product of staple sg's upper term nu = sg + 1, to the next
+    // staple: its adjoint is the first product of that staple's deferred
+    // term, which therefore costs no loads.
+    Mat<R, NC> keep;
+    auto build = [&](int sg, int defer) -> Mat<R, NC> {
+      Mat<R, NC> acc = zero<R, NC>();
+      const long long s_xp = idx_shift<ND>(g, x, sg, 1, -1, 0);
+      auto upper = [&](int j) {
+        const int nu = j < sg ? j : j + 1;
+        if (nu == defer) return;
+        const Mat<R, NC> ab = mul_ad(f.load(s_xp, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), sg));
+        if (nu == sg + 1) keep = ab;
+        add_to(acc, mul_ad(ab, f.load(s, nu)));
+      };
+      auto lower = [&](int j) {
+        const int nu = j < sg ? j : j + 1;
+        const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+        add_to(acc, mul(mul_dd(f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu), f.load(s_xmnu, sg)),
+                        f.load(s_xmnu, nu)));
+      };
+      if constexpr (ILV) {
+        // upper and lower term of one nu back to back: the lower term's
+        // U_sg(x-nu) is the upper term's U_sg(x'+nu) of the site x' = x - 2 nu,
+        // a neighbouring lane or wave, so the second read tends to hit L2
+#pragma unroll 1
+        for (int j = 0; j < ND - 1; ++j) {
+          upper(j);
+          lower(j);
+        }
+      } else {
+#pragma unroll 1
+        for (int j = 0; j < ND - 1; ++j) upper(j);
+#pragma unroll 1
+        for (int j = 0; j < ND - 1; ++j) lower(j);
+      }
+      return acc;
+    };
+    if (active) {
+      lds_put<R, NC>(bS, build(0, -1));
+      lds_put<R, NC>(bU, f.load(s, 0));
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int mu = 0; mu < ND; ++mu) {
+      const int m = mu + 1;
+      __syncthreads();
+      if (active && m < ND) {
+        lds_put<R, NC>(bA, adjoint(keep));
+        lds_put<R, NC>(bS, build(m, mu));
+        lds_put<R, NC>(bU, f.load(s, m));
+      }
+      __syncthreads();
+    }
+  }
+  n = wave_sum_u64(n);
+  if ((threadIdx.x & 63) == 0) red[w] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+#pragma unroll
+    for (int i = 0; i < kWsBlock / 64; ++i) b += red[i];
+    if (b) atomicAdd(nacc, b);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Group variants: four (quad) or two (pair) lanes per site.  The proposals
 // (and their accept draws) do not depend on the field, so for hits
 // h0..h0+LANES-1 lane q generates the proposal of hit h0+q from its own keyed
@@ -252,8 +457,9 @@ __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_group(
 // 4 (one lane, low-register staple, occupancy 2), 5 (one lane, proposals
 // drawn in pairs), 6 (one lane, low-register staple, pairs), 7 (one lane,
 // next staple loaded during the hits), 8 (as 7, FP64 links staged in LDS)
-// 9, 10 (pair per site, occupancy 2, 4) or 11 (one lane, own links kept
-// in registers)
+// 9, 10 (pair per site, occupancy 2, 4), 11 (one lane, own links kept
+// in registers), 12 (site waves fed by staple waves) or 13 (as 12, each
+// direction's upper and lower staple terms back to back)
 template <typename RR, bool CC>
 int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1,
                         const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
@@ -270,7 +476,15 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
     const unsigned nb = blocks_for(f->g.Vh * (quad ? 4 : (pair ? 2 : 1)), kMetroBlock);
     for (int pass = p0; pass < p1; ++pass) {
       ProfScope prof("metropolis_pass");
-      if (quad)
+      if (variant == 12)
+        hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, false>), dim3(blocks_for(f->g.Vh, kWsSites)),
+                           dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
+                           (uint32_t)sweep, chain, counter, visit);
+      else if (variant == 13)
+        hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, true>), dim3(blocks_for(f->g.Vh, kWsSites)),
+                           dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
+                           (uint32_t)sweep, chain, counter, visit);
+      else if (quad)
         with_occ(variant - 1, [&](auto occ) -> int {
           constexpr int OCC = decltype(occ)::value;
           hipLaunchKernelGGL((k_metropolis_group<R, NC, COMP, ND, OCC, 4>), dim3(nb), dim3(kMetroBlock), 0,

@@ -21,7 +21,7 @@
 using namespace kd;
 namespace ki = klft_internal;
 
-constexpr int kMetroVariants = 12;
+constexpr int kMetroVariants = 14;
 
 namespace klft_internal {
 int launch_passes_f64(klft_field_t f, void *links, int variant, int p0, int p1, const klft_metropolis_params *p,
@@ -68,10 +68,14 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   // staple; 7 one lane building the next direction's staple during the hits;
   // 8 the same with FP64 links staged through LDS (7 for FP32 storage);
   // 9, 10 a pair of lanes per site with >= 2, 4 waves/SIMD; 11 one lane
-  // with the site's own links kept in registers.
-  // Untuned defaults, the fastest at 32^4 (nHits = 10): LDS-staged
-  // pipelined staple for FP64 SU(3), own links in registers otherwise.
-  const int dflt = (f->nc == 3 && f->storage == KLFT_STORAGE_F64) ? 8 : 11;
+  // with the site's own links kept in registers; 12 site waves fed the next
+  // staple by staple waves; 13 the same with each direction's upper and
+  // lower staple terms back to back.
+  // Untuned defaults: site waves fed by staple waves, the fastest at 16^4,
+  // 24^4 and 32^4 for every group (nHits = 10), with the interleaved term
+  // order for FP64 links and the upper-terms-first order for FP32 links
+  // (the interleaved order loses at 48^4 in FP32).
+  const int dflt = f->storage == KLFT_STORAGE_F64 ? 13 : 12;
   const int v = pick_variant("metropolis", f, kMetroVariants, dflt, [&](int var) -> int {
     if (!scratch) {
       KLFT_HIP(hipMalloc(&scratch, f->link_bytes));

@@ -184,7 +184,7 @@ def test_fp32_plaquette_within_1e6(nc, storage):
     assert abs(got - ref) < 1e-6
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 @pytest.mark.parametrize("nc,beta,storage", [(1, 2.0, "f64"), (2, 2.3, "f64"), (3, 5.5, "f64"),
                                              (3, 5.5, "f32"), (2, 2.3, "f32_rows")])
 def test_metropolis_variants_agree(monkeypatch, variant, nc, beta, storage):
@@ -192,8 +192,9 @@ def test_metropolis_variants_agree(monkeypatch, variant, nc, beta, storage):
     targets; one lane with the low-register staple and/or proposals drawn in
     pairs; one lane building the next staple during the hits, in registers or
     staged through LDS; a pair of lanes per site at two occupancy targets; one
-    lane with the site's own links in registers) makes the same decisions: FP64 against the oracle, FP32 against
-    variant 0."""
+    lane with the site's own links in registers; site waves fed by staple
+    waves, in two staple term orders) makes the same decisions: FP64 against
+    the oracle, FP32 against variant 0."""
     nd, dims = 4, (4, 6, 4, 4)
     U = orc.random_algebra(nd, nc, dims, 0.3, 19)
     p = klft.MetropolisParams(nHits=7, beta=beta, delta=0.3, seed=99)
@@ -210,6 +211,26 @@ def test_metropolis_variants_agree(monkeypatch, variant, nc, beta, storage):
         g0 = klft.GaugeField(nd, nc, dims, storage).upload(U)
         n_0 = [klft.sweep_Metropolis(g0, p, s, with_count=True)[1] for s in range(2)]
         assert abs(n_g[0] - n_0[0]) <= 2 and abs(n_g[1] - n_0[1]) <= 4
+
+
+@pytest.mark.parametrize("nd,nc,dims", [(4, 3, (12, 12, 8, 10)), (4, 2, (8, 8, 8, 8)), (3, 3, (10, 8, 12)),
+                                        (2, 1, (16, 12)), (4, 1, (6, 6, 6, 6))])
+@pytest.mark.parametrize("variant", [12, 13])
+def test_staple_wave_sweep_matches_oracle(monkeypatch, nd, nc, dims, variant):
+    """The site-wave / staple-wave kernel on lattices with several blocks and
+    a partly filled last block (every wave of a block meets every barrier),
+    over three sweeps: FP64 decisions identical to the oracle, fields within
+    1e-11."""
+    U = orc.random_algebra(nd, nc, dims, 0.4, 71)
+    beta = {1: 2.0, 2: 2.3, 3: 5.5}[nc]
+    p = klft.MetropolisParams(nHits=6, beta=beta, delta=0.3, seed=123)
+    monkeypatch.setenv("KLFT_VARIANT", f"metropolis={variant}")
+    g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
+    n_g = [klft.sweep_Metropolis(g, p, s, with_count=True)[1] for s in range(3)]
+    Uo = U.copy()
+    n_o = [orc.metropolis_sweep(Uo, nd, nc, dims, 6, beta, 0.3, 123, s)[1] for s in range(3)]
+    assert n_g == n_o
+    np.testing.assert_allclose(g.download(), Uo, rtol=0, atol=1e-11)
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
