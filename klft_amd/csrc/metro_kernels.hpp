@@ -148,6 +148,17 @@ __device__ __forceinline__ Mat<R, NC> lds_get(const cx<R> *buf) {
   return a;
 }
 
+// store the link held in an LDS buffer, one component at a time (a whole
+// matrix in registers at this point would spill)
+template <typename R, int NC, bool COMP>
+__device__ __forceinline__ void store_from_lds(const Links<R, NC, COMP> &f, long long s, int mu,
+                                               const cx<R> *buf) {
+  cx<R> *q = f.p + f.base(s, mu);
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < Links<R, NC, COMP>::NCOMP; ++c) q[c * 64] = buf[c * 64 + l];
+}
+
 // LinkChain with U in the site wave's LDS buffer `bu`: U is read only when
 // an accepted proposal is multiplied in, and two waves per SIMD leave each
 // wave half the registers of the one-wave kernels.
@@ -246,7 +257,7 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
           lc.step(r, u, beta_nc, n);
         }
         lc.apply_pending();
-        if (nhits > 0) f.store(s, mu, lds_get<R, NC>(lc.bu));
+        if (nhits > 0) store_from_lds(f, s, mu, lc.bu);
       }
       __syncthreads();
     }
