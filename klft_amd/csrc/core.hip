@@ -489,7 +489,8 @@ extern "C" int klft_field_reunitarize(klft_field_t f) {
 // plaquette (GaugePlaquette.hpp:36-135)
 // ============================================================================
 // variants 0, 1, 2: plane by plane at >= 1, 2, 4 waves/SIMD; 3: own links
-// loaded once, all planes' loads in flight together
+// loaded once, all planes' loads in flight together (4, 5: k_plaquette_walk,
+// below, all loads in flight / plane by plane)
 template <typename R, int NC, bool COMP, int ND, int OCC, bool FULL = false>
 __global__ __launch_bounds__(256, OCC) void k_plaquette(Links<R, NC, COMP> f, Geom g, double *partials,
                                                     unsigned int *ticket, double *out,
@@ -510,11 +511,76 @@ __global__ __launch_bounds__(256, OCC) void k_plaquette(Links<R, NC, COMP> f, Ge
   grid_finish<256>(bt, partials, ticket, out, lds);
 }
 
+// variants 4, 5: each thread walks S consecutive sites along dimension 0 at a
+// fixed position in the other dimensions.  A block holds 256 consecutive
+// such positions (last dimension fastest) of one segment of dimension 0, and
+// all segments run at once, so the neighbours a site reads in dimensions
+// 1..nd-1 are read by the same or adjacent blocks in the same slice at about
+// the same time, and its neighbour in dimension 0 is the site the same thread
+// reads next: each link is re-read while it is still in L2.
+// FULL: all planes' loads in flight together (else plane by plane, fewer
+// registers and more waves per SIMD)
+template <typename R, int NC, bool COMP, int ND, bool FULL>
+__global__ __launch_bounds__(256) void k_plaquette_walk(Links<R, NC, COMP> f, Geom g, int S, long long Vs,
+                                                        double *partials, unsigned int *ticket, double *out,
+                                                        double *per_site) {
+  __shared__ double lds[8];
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned int nreg = (unsigned int)((Vs + 255) / 256);
+  const int seg = (int)(lb / nreg);
+  const long long sp = (long long)(lb % nreg) * 256 + threadIdx.x;
+  double v = 0.0;
+  if (sp < Vs) {
+    int x[ND];
+    long long r = sp;
+#pragma unroll
+    for (int d = ND - 1; d >= 1; --d) {
+      x[d] = (int)(r % g.L[d]);
+      r /= g.L[d];
+    }
+    const int t0 = seg * S;
+#pragma unroll 1
+    for (int j = 0; j < S && t0 + j < g.L[0]; ++j) {
+      x[0] = t0 + j;
+      const long long s = dev_index<ND>(g, x);
+      double p;
+      if constexpr (FULL)
+        p = plaq_at_full<R, NC, COMP, ND>(f, make_nbr<ND>(g, x, s));
+      else
+        p = plaq_at<R, NC, COMP, ND>(f, g, x, s);
+      if (per_site) per_site[lex_index<ND>(g, x)] = p;
+      v += p;
+    }
+  }
+  const double bt = block_sum<256>(v, lds);
+  grid_finish<256>(bt, partials, ticket, out, lds);
+}
+
 namespace klft_internal {
 // raw (unnormalised) plaquette sum into f->result[slot]; optional per-site
 static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int variant) {
   return dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
+    if (variant == 4 || variant == 5) {
+      // segments of dimension 0 enough for about 2^18 threads
+      const long long Vs = f->g.V / f->g.L[0];
+      const int nseg = (int)std::min<long long>(f->g.L[0], std::max<long long>(1, ((1LL << 18) + Vs - 1) / Vs));
+      const int S = (f->g.L[0] + nseg - 1) / nseg;
+      const int nsegs = (f->g.L[0] + S - 1) / S;
+      const unsigned nb = (unsigned)(nsegs * ((Vs + 255) / 256));
+      if ((long long)nb > f->max_partials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
+      ProfScope prof("plaquette");
+      if (variant == 4)
+        hipLaunchKernelGGL((k_plaquette_walk<R, NC, COMP, ND, true>), dim3(nb), dim3(256), 0, stream(),
+                           links_of<R, NC, COMP>(f), f->g, S, Vs, f->partials, f->ticket, f->result + slot,
+                           dev_per_site);
+      else
+        hipLaunchKernelGGL((k_plaquette_walk<R, NC, COMP, ND, false>), dim3(nb), dim3(256), 0, stream(),
+                           links_of<R, NC, COMP>(f), f->g, S, Vs, f->partials, f->ticket, f->result + slot,
+                           dev_per_site);
+      KLFT_HIP(hipGetLastError());
+      return KLFT_OK;
+    }
     const unsigned nb = blocks_for(chunk_threads<ND>(f->g), 256);
     if ((long long)nb > f->max_partials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
     if (variant == 3) {
@@ -536,8 +602,9 @@ static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int 
   });
 }
 int plaquette_raw(klft_field_t f, int slot, double *dev_per_site) {
-  // untuned default, the fastest at 32^4: all planes at once for FP32 links
-  const int v = pick_variant("plaquette", f, kOccVariants + 1, f->storage == KLFT_STORAGE_F64 ? 0 : 3,
+  // untuned defaults, the fastest at 32^4 and 48^4: threads walking
+  // dimension 0, plane by plane for FP64 links, all loads in flight for FP32
+  const int v = pick_variant("plaquette", f, kOccVariants + 3, f->storage == KLFT_STORAGE_F64 ? 5 : 4,
                              [&](int var) { return launch_plaquette(f, slot, nullptr, var); });
   return launch_plaquette(f, slot, dev_per_site, v);
 }

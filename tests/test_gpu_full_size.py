@@ -97,3 +97,21 @@ def test_lattice_beyond_65536_reduction_blocks():
     h8 = klft.HMC(klft.GaugeField(4, 3, small, "f64").upload(U8), beta=6.0, tau=1.0, nsteps=20, seed=5)
     h8.set_momenta(P8)
     assert abs(h.kinetic() - 1080 * h8.kinetic()) <= 1e-12 * abs(h.kinetic())
+
+
+@pytest.mark.parametrize("variant", [4, 5])
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_walking_plaquette_with_a_partial_segment(monkeypatch, storage, variant):
+    """The dimension-0 walking plaquette (variants 4, 5) on 26 x 32^3: 32,768
+    walkers per segment, 4 sites per walker, the last of 7 segments holding 2.
+    Per-site values against the oracle (FP64 1e-12, FP32 links 5e-5 per site)
+    and the sum."""
+    dims = (26, 32, 32, 32)
+    U = orc.random_algebra(4, 3, dims, 0.5, 808)
+    monkeypatch.setenv("KLFT_VARIANT", f"plaquette={variant}")
+    g = klft.GaugeField(4, 3, dims, storage).upload(U)
+    got, ps = klft.GaugePlaquette(g, per_site=True)
+    want, ps_o = orc.plaquette(U, 4, 3, dims, per_site=True)
+    tol = 1e-12 if storage == "f64" else 5e-5
+    np.testing.assert_allclose(ps, ps_o, rtol=0, atol=tol)
+    assert abs(got - want) < (1e-12 if storage == "f64" else 1e-6)

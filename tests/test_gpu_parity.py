@@ -233,15 +233,18 @@ def test_staple_wave_sweep_matches_oracle(monkeypatch, nd, nc, dims, variant):
     np.testing.assert_allclose(g.download(), Uo, rtol=0, atol=1e-11)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("nd,nc,dims", [(4, 3, (4, 4, 6, 4)), (3, 2, (4, 6, 4)), (4, 1, (6, 4, 4, 4))])
 def test_measurement_and_force_variants_match_oracle(monkeypatch, variant, nd, nc, dims):
-    """Every plaquette and staple-field variant (occupancy 1, 2, 4, and all
-    loads in flight at once) and force variant (one thread per link at three
+    """Every plaquette variant (occupancy 1, 2, 4, all loads in flight at
+    once, threads walking dimension 0 with all loads in flight or plane by
+    plane), staple-field variant (occupancy 1, 2,
+    4, all loads in flight) and force variant (one thread per link at three
     occupancies, one thread per site, one thread per link with the full
     staple) against the oracle."""
     U = seeded(nd, nc, dims, 0.6, 41)
-    monkeypatch.setenv("KLFT_VARIANT", f"plaquette={min(variant, 3)},staple_field={min(variant, 3)},force={variant}")
+    monkeypatch.setenv("KLFT_VARIANT",
+                       f"plaquette={variant},staple_field={min(variant, 3)},force={min(variant, 4)}")
     g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
     got, ps = klft.GaugePlaquette(g, per_site=True)
     want, ps_o = orc.plaquette(U, nd, nc, dims, per_site=True)
