@@ -233,18 +233,68 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
   }
   cx<R> *bS = stage[wi][0], *bA = stage[wi][1], *bU = stage[wi][2];
   unsigned long long n = 0;
+  // Staple sg of the site without its upper term nu = defer, restricted to
+  // the upper terms j with bit j of umask and the lower terms j with bit j of
+  // lmask (j-th direction other than sg).  keep (if given) receives
+  // U_{sg+1}(x+sg) U_sg(x+sg+1)^dagger, the first product of the upper term
+  // nu = sg + 1: its adjoint is the first product of the next staple's
+  // deferred term, which therefore costs no loads.
+  auto build = [&](int sg, int defer, unsigned umask, unsigned lmask, Mat<R, NC> *keep) -> Mat<R, NC> {
+    Mat<R, NC> acc = zero<R, NC>();
+    const long long s_xp = idx_shift<ND>(g, x, sg, 1, -1, 0);
+    auto upper = [&](int j) {
+      const int nu = j < sg ? j : j + 1;
+      if (nu == defer || !((umask >> j) & 1u)) return;
+      const Mat<R, NC> ab = mul_ad(f.load(s_xp, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), sg));
+      if (keep && nu == sg + 1) *keep = ab;
+      add_to(acc, mul_ad(ab, f.load(s, nu)));
+    };
+    auto lower = [&](int j) {
+      if (!((lmask >> j) & 1u)) return;
+      const int nu = j < sg ? j : j + 1;
+      const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+      add_to(acc, mul(mul_dd(f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu), f.load(s_xmnu, sg)),
+                      f.load(s_xmnu, nu)));
+    };
+    if constexpr (ILV) {
+      // upper and lower term of one nu back to back: the lower term's
+      // U_sg(x-nu) is the upper term's U_sg(x'+nu) of the site x' = x - 2 nu,
+      // a neighbouring lane or wave, so the second read tends to hit L2
+#pragma unroll 1
+      for (int j = 0; j < ND - 1; ++j) {
+        upper(j);
+        lower(j);
+      }
+    } else {
+#pragma unroll 1
+      for (int j = 0; j < ND - 1; ++j) upper(j);
+#pragma unroll 1
+      for (int j = 0; j < ND - 1; ++j) lower(j);
+    }
+    return acc;
+  };
+  // The first staple of a pass has no hits to hide behind, so the two waves
+  // split it: the staple wave takes the upper term nu = 1 (which gives keep)
+  // and the lower terms j >= 1, the site wave the rest, adding its part to
+  // the handed-over one.
+  constexpr unsigned kAll = (1u << (ND - 1)) - 1u;
   // the two roles run in separate branches, so neither role's registers are
   // held across the other's code; each executes 1 + 2 nd barriers
   if (sitew) {
     if (active && visit) visit[site] = pass;
     LdsChain<R, NC> lc;
     lc.bu = stage[wi][3];
+    // the site wave's part of the first staple waits in its own buffer
+    if (active) lds_put<R, NC>(lc.bu, build(0, -1, kAll & ~1u, 1u, nullptr));
     __syncthreads();
 #pragma unroll 1
     for (int mu = 0; mu < ND; ++mu) {
       if (active) {
         Mat<R, NC> S = lds_get<R, NC>(bS);
-        if (mu > 0) add_to(S, mul_ad(lds_get<R, NC>(bA), lds_get<R, NC>(lc.bu)));  // the final U_{mu-1}(x)
+        if (mu > 0)
+          add_to(S, mul_ad(lds_get<R, NC>(bA), lds_get<R, NC>(lc.bu)));  // the final U_{mu-1}(x)
+        else
+          add_to(S, lds_get<R, NC>(lc.bu));
         lc.start(S, lds_get<R, NC>(bU));
       }
       __syncthreads();
@@ -262,46 +312,9 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
       __syncthreads();
     }
   } else {
-    // staple waves.  keep carries U_{sg+1}(x+sg) U_sg(x+sg+1)^dagger, the
-    // first product of staple sg's upper term nu = sg + 1, to the next
-    // staple: its adjoint is the first product of that staple's deferred
-    // term, which therefore costs no loads.
     Mat<R, NC> keep;
-    auto build = [&](int sg, int defer) -> Mat<R, NC> {
-      Mat<R, NC> acc = zero<R, NC>();
-      const long long s_xp = idx_shift<ND>(g, x, sg, 1, -1, 0);
-      auto upper = [&](int j) {
-        const int nu = j < sg ? j : j + 1;
-        if (nu == defer) return;
-        const Mat<R, NC> ab = mul_ad(f.load(s_xp, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), sg));
-        if (nu == sg + 1) keep = ab;
-        add_to(acc, mul_ad(ab, f.load(s, nu)));
-      };
-      auto lower = [&](int j) {
-        const int nu = j < sg ? j : j + 1;
-        const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
-        add_to(acc, mul(mul_dd(f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu), f.load(s_xmnu, sg)),
-                        f.load(s_xmnu, nu)));
-      };
-      if constexpr (ILV) {
-        // upper and lower term of one nu back to back: the lower term's
-        // U_sg(x-nu) is the upper term's U_sg(x'+nu) of the site x' = x - 2 nu,
-        // a neighbouring lane or wave, so the second read tends to hit L2
-#pragma unroll 1
-        for (int j = 0; j < ND - 1; ++j) {
-          upper(j);
-          lower(j);
-        }
-      } else {
-#pragma unroll 1
-        for (int j = 0; j < ND - 1; ++j) upper(j);
-#pragma unroll 1
-        for (int j = 0; j < ND - 1; ++j) lower(j);
-      }
-      return acc;
-    };
     if (active) {
-      lds_put<R, NC>(bS, build(0, -1));
+      lds_put<R, NC>(bS, build(0, -1, 1u, kAll & ~1u, &keep));
       lds_put<R, NC>(bU, f.load(s, 0));
     }
     __syncthreads();
@@ -311,7 +324,7 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
       __syncthreads();
       if (active && m < ND) {
         lds_put<R, NC>(bA, adjoint(keep));
-        lds_put<R, NC>(bS, build(m, mu));
+        lds_put<R, NC>(bS, build(m, mu, kAll, kAll, &keep));
         lds_put<R, NC>(bU, f.load(s, m));
       }
       __syncthreads();
