@@ -120,15 +120,16 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe_lds(
 // 4 site waves and 4 staple waves, two waves per SIMD.  Site wave i runs the
 // Markov chains of its 64 sites; staple wave i + 4 builds the staple of the
 // next direction for the same sites and hands it over through LDS, so the
-// SIMD computes hits while the other wave waits on memory.  The staple is the
-// pipelined kernel's (same terms, same order): every term except the upper
-// one through U_mu(x), which the site wave adds once U_mu(x) is final, from
-// U_mu(x+m) U_m(x+mu)^dagger left by the staple wave.  The site's own link
-// is handed over the same way.  Per direction the block meets two barriers:
+// SIMD computes hits while the other wave waits on memory.  The staple has
+// the pipelined kernel's terms: every term except the upper one through
+// U_mu(x), which the site wave adds once U_mu(x) is final, from
+// U_mu(x+m) U_m(x+mu)^dagger left by the staple wave (with ILV each nu's
+// upper and lower terms are taken back to back).  The site's own link is
+// handed over the same way.  Per direction the block meets two barriers:
 // after the site waves have read the handover buffers, and after the staple
 // waves have refilled them and the site waves have stored the updated link
-// (which the next staple reads as U_nu(x), nu < m).  Every wave of the block
-// runs the same loop, so every barrier is met by all eight.
+// (which the next staple reads as U_nu(x), nu < m).  Both roles meet the same
+// 1 + 2 nd barriers, lanes past the sublattice included.
 // ---------------------------------------------------------------------------
 constexpr int kWsBlock = 512;
 constexpr int kWsSites = 256;
