@@ -25,6 +25,7 @@
 
 #include "internal.hpp"
 #include "metro.hpp"
+#include "metro_kernels.hpp"
 
 using namespace kd;
 namespace ki = klft_internal;
@@ -86,6 +87,47 @@ __global__ __launch_bounds__(kB) void k_batch_pass(const ChainDev *__restrict__ 
     for (int i = 0; i < kB / 64; ++i) b += red[i];
     if (b) atomicAdd(&counters[c], b);
   }
+}
+
+// The same pass with site waves fed by staple waves (klft_metro::ws_pass):
+// block b of the work table holds the 256 positions of chain block_chain[b]
+// that k_batch_pass gives it, in 8 waves.  The group is uniform per block, so
+// all eight waves meet the same barriers.
+template <typename R, bool COMP, bool DEFECT>
+__global__ __launch_bounds__(klft_metro::kWsBlock) void k_batch_ws(
+    const ChainDev *__restrict__ chains, const int *__restrict__ block_chain, int pass, int nhits, R delta,
+    uint64_t seed, const uint32_t *__restrict__ d_sweep, unsigned long long *counters, Defect d,
+    const double *__restrict__ cval) {
+  __shared__ cx<R> stage[4 * 4 * 9 * 64];
+  __shared__ unsigned long long red[klft_metro::kWsBlock / 64];
+  const int c = block_chain[blockIdx.x];
+  const ChainDev ch = chains[c];
+  const long long t = (long long)(blockIdx.x - ch.first_block) * kB + (long long)((threadIdx.x >> 6) & 3) * 64 +
+                      (threadIdx.x & 63);
+  const uint32_t sweep = *d_sweep;
+  const R cc = DEFECT ? (R)cval[c] : R(1);
+  unsigned long long n = 0;
+  switch (ch.nc) {
+    case 1: {
+      Links<R, 1, false> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
+      n = klft_metro::ws_pass<R, 1, false, 4, false, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 1.0), delta,
+                                                             seed, sweep, (uint32_t)c, nullptr, d, cc, stage);
+      break;
+    }
+    case 2: {
+      Links<R, 2, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
+      n = klft_metro::ws_pass<R, 2, COMP, 4, false, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 2.0), delta,
+                                                            seed, sweep, (uint32_t)c, nullptr, d, cc, stage);
+      break;
+    }
+    default: {
+      Links<R, 3, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
+      n = klft_metro::ws_pass<R, 3, COMP, 4, false, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 3.0), delta,
+                                                            seed, sweep, (uint32_t)c, nullptr, d, cc, stage);
+      break;
+    }
+  }
+  klft_metro::ws_count(n, red, &counters[c]);
 }
 
 template <typename R, bool COMP>
@@ -272,8 +314,19 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
   auto passes = [&](auto rt, auto ct) -> int {
     using R = typename decltype(rt)::type;
     constexpr bool COMP = decltype(ct)::value != 0;
+    // variant 0: one lane per site (k_batch_pass); 1 (default): site waves
+    // fed by staple waves (k_batch_ws)
+    const bool ws = ki::forced_variant("batch") != 0;
     for (int pass = 0; pass < 16; ++pass) {
-      if (b->ptbc)
+      if (ws && b->ptbc)
+        hipLaunchKernelGGL((k_batch_ws<R, COMP, true>), dim3(b->nblocks), dim3(klft_metro::kWsBlock), 0,
+                           ki::stream(), b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep,
+                           b->d_counters, b->defect, b->d_cval);
+      else if (ws)
+        hipLaunchKernelGGL((k_batch_ws<R, COMP, false>), dim3(b->nblocks), dim3(klft_metro::kWsBlock), 0,
+                           ki::stream(), b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep,
+                           b->d_counters, b->defect, b->d_cval);
+      else if (b->ptbc)
         hipLaunchKernelGGL((k_batch_pass<R, COMP, true>), dim3(b->nblocks), dim3(kB), 0, ki::stream(),
                            b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters,
                            b->defect, b->d_cval);

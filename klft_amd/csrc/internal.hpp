@@ -86,6 +86,8 @@ bool tuning_enabled();
 std::string tune_key(const char *kernel, const klft_field_s *f);
 int pick_variant(const char *kernel, const klft_field_s *f, int nvar, int dflt,
                  const std::function<int(int)> &run);
+// the variant KLFT_VARIANT forces for `kernel` ("name=v,..."), or -1
+int forced_variant(const char *kernel);
 
 // occupancy variants: v = 0, 1, 2 -> at least 1, 2, 4 waves per SIMD
 constexpr int kOccVariants = 3;

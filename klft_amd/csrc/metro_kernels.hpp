@@ -184,8 +184,8 @@ struct LdsChain {
       pending = false;
     }
   }
-  __device__ __forceinline__ void step(const Mat<R, NC> &r, R u, R beta_nc, unsigned long long &n) {
-    if (metro_accept(-beta_nc * (retr_mul(r, M) - trM), u)) {
+  __device__ __forceinline__ void step(const Mat<R, NC> &r, R u, R beta_nc, R cl, unsigned long long &n) {
+    if (metro_accept(cl * (-beta_nc * (retr_mul(r, M) - trM)), u)) {
       apply_pending();
       P = r;
       pending = true;
@@ -208,22 +208,29 @@ __device__ __forceinline__ Mat<R, NC> adjoint(const Mat<R, NC> &a) {
 
 // ILV: each nu's upper and lower staple terms back to back (else all upper
 // terms first, the order of the other kernels)
-template <typename R, int NC, bool COMP, int ND, bool ILV>
-__global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
-    Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
-    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+// One pass of position t = wi * 64 + lane of the block's 256 sites (t past
+// the sublattice: no work, every barrier met), for the calling wave's role;
+// returns the site wave's accepted hits.  `stage` holds, per site wave, the
+// staple (all terms but the deferred one), the deferred term's first
+// product, the own link of the next direction, and the link being updated:
+// 4 x 4 x Nc^2 x 64 complex.  With DEFECT every link read is scaled by c on
+// the defect and the updated link's own factor multiplies dS, as in
+// metro_site.
+template <typename R, int NC, bool COMP, int ND, bool ILV, bool DEFECT>
+__device__ __forceinline__ unsigned long long ws_pass(const Links<R, NC, COMP> &f, const Geom &g, int pass,
+                                                      long long t, int nhits, R beta_nc, R delta, uint64_t seed,
+                                                      uint32_t sweep, uint32_t chain, int *visit, const Defect &d,
+                                                      R c, cx<R> *stage_base) {
   constexpr int NM = NC * NC;
-  // per site wave: the staple (all terms but the deferred one), the deferred
-  // term's first product, the own link of the next direction, and the link
-  // being updated
-  __shared__ cx<R> stage[4][4][NM * 64];
-  __shared__ unsigned long long red[kWsBlock / 64];
   const int w = (int)(threadIdx.x >> 6);
   const int wi = w & 3;
   const bool sitew = w < 4;
-  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
-  const long long t = (long long)lb * kWsSites + wi * 64 + (threadIdx.x & 63);
   const bool active = t < g.Vh;
+  auto stage = [&](int k) { return stage_base + (wi * 4 + k) * NM * 64; };
+  auto cl_of = [&](const int *y, int mu) -> R {
+    if constexpr (DEFECT) return on_defect<ND>(g, d, y, mu) ? c : R(1);
+    return R(1);
+  };
   int x[ND];
   long long s = 0;
   uint32_t site = 0;
@@ -232,7 +239,7 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
     s = (long long)pass * g.Vh + t;
     site = (uint32_t)lex_index<ND>(g, x);
   }
-  cx<R> *bS = stage[wi][0], *bA = stage[wi][1], *bU = stage[wi][2];
+  cx<R> *bS = stage(0), *bA = stage(1), *bU = stage(2);
   unsigned long long n = 0;
   // Staple sg of the site without its upper term nu = defer, restricted to
   // the upper terms j with bit j of umask and the lower terms j with bit j of
@@ -243,19 +250,36 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
   auto build = [&](int sg, int defer, unsigned umask, unsigned lmask, Mat<R, NC> *keep) -> Mat<R, NC> {
     Mat<R, NC> acc = zero<R, NC>();
     const long long s_xp = idx_shift<ND>(g, x, sg, 1, -1, 0);
+    // the link at x + s1 e_m1 + s2 e_m2 in direction dir, defect-scaled
+    auto ld = [&](int m1, int s1, int m2, int s2, int dir) -> Mat<R, NC> {
+      int y[ND];
+      shifted<ND>(g, x, m1, s1, m2, s2, y);
+      return load_at<R, NC, COMP, ND, true>(f, g, y, dir, d, c);
+    };
     auto upper = [&](int j) {
       const int nu = j < sg ? j : j + 1;
       if (nu == defer || !((umask >> j) & 1u)) return;
-      const Mat<R, NC> ab = mul_ad(f.load(s_xp, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), sg));
+      Mat<R, NC> ab;
+      if constexpr (DEFECT)
+        ab = mul_ad(ld(sg, 1, -1, 0, nu), ld(nu, 1, -1, 0, sg));
+      else
+        ab = mul_ad(f.load(s_xp, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), sg));
       if (keep && nu == sg + 1) *keep = ab;
-      add_to(acc, mul_ad(ab, f.load(s, nu)));
+      if constexpr (DEFECT)
+        add_to(acc, mul_ad(ab, ld(-1, 0, -1, 0, nu)));
+      else
+        add_to(acc, mul_ad(ab, f.load(s, nu)));
     };
     auto lower = [&](int j) {
       if (!((lmask >> j) & 1u)) return;
       const int nu = j < sg ? j : j + 1;
-      const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
-      add_to(acc, mul(mul_dd(f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu), f.load(s_xmnu, sg)),
-                      f.load(s_xmnu, nu)));
+      if constexpr (DEFECT) {
+        add_to(acc, mul(mul_dd(ld(sg, 1, nu, -1, nu), ld(nu, -1, -1, 0, sg)), ld(nu, -1, -1, 0, nu)));
+      } else {
+        const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+        add_to(acc, mul(mul_dd(f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu), f.load(s_xmnu, sg)),
+                        f.load(s_xmnu, nu)));
+      }
     };
     if constexpr (ILV) {
       // upper and lower term of one nu back to back: the lower term's
@@ -284,7 +308,7 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
   if (sitew) {
     if (active && visit) visit[site] = pass;
     LdsChain<R, NC> lc;
-    lc.bu = stage[wi][3];
+    lc.bu = stage(3);
     // the site wave's part of the first staple waits in its own buffer
     if (active) lds_put<R, NC>(lc.bu, build(0, -1, kAll & ~1u, 1u, nullptr));
     __syncthreads();
@@ -292,20 +316,31 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
     for (int mu = 0; mu < ND; ++mu) {
       if (active) {
         Mat<R, NC> S = lds_get<R, NC>(bS);
-        if (mu > 0)
-          add_to(S, mul_ad(lds_get<R, NC>(bA), lds_get<R, NC>(lc.bu)));  // the final U_{mu-1}(x)
-        else
+        if (mu > 0) {
+          // the final U_{mu-1}(x), defect-scaled
+          Mat<R, NC> Up = lds_get<R, NC>(lc.bu);
+          if constexpr (DEFECT) {
+            const R cp = cl_of(x, mu - 1);
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+#pragma unroll
+              for (int k = 0; k < NC; ++k) Up.m[i][k] = scale(cp, Up.m[i][k]);
+          }
+          add_to(S, mul_ad(lds_get<R, NC>(bA), Up));
+        } else {
           add_to(S, lds_get<R, NC>(lc.bu));
+        }
         lc.start(S, lds_get<R, NC>(bU));
       }
       __syncthreads();
       if (active) {
+        const R cl = cl_of(x, mu);
 #pragma unroll 1
         for (int hit = 0; hit < nhits; ++hit) {
           Mat<R, NC> r;
           R u;
           propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
-          lc.step(r, u, beta_nc, n);
+          lc.step(r, u, beta_nc, cl, n);
         }
         lc.apply_pending();
         if (nhits > 0) store_from_lds(f, s, mu, lc.bu);
@@ -331,8 +366,13 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
       __syncthreads();
     }
   }
+  return n;
+}
+
+// block total of the site waves' accepted hits into *nacc
+__device__ __forceinline__ void ws_count(unsigned long long n, unsigned long long *red, unsigned long long *nacc) {
   n = wave_sum_u64(n);
-  if ((threadIdx.x & 63) == 0) red[w] = n;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long b = 0;
@@ -340,6 +380,19 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
     for (int i = 0; i < kWsBlock / 64; ++i) b += red[i];
     if (b) atomicAdd(nacc, b);
   }
+}
+
+template <typename R, int NC, bool COMP, int ND, bool ILV>
+__global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
+    Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
+    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  __shared__ cx<R> stage[4 * 4 * NC * NC * 64];
+  __shared__ unsigned long long red[kWsBlock / 64];
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long t = (long long)lb * kWsSites + (long long)((threadIdx.x >> 6) & 3) * 64 + (threadIdx.x & 63);
+  const unsigned long long n = ws_pass<R, NC, COMP, ND, ILV, false>(f, g, pass, t, nhits, beta_nc, delta, seed,
+                                                                    sweep, chain, visit, Defect{}, R(1), stage);
+  ws_count(n, red, nacc);
 }
 
 // ---------------------------------------------------------------------------

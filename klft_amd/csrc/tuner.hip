@@ -52,7 +52,7 @@ std::string tune_key(const char *kernel, const klft_field_s *f) {
   return k;
 }
 
-static int forced_variant(const char *kernel) {
+int forced_variant(const char *kernel) {
   const char *e = std::getenv("KLFT_VARIANT");
   if (!e) return -1;
   const std::string s(e), name(kernel);
