@@ -562,9 +562,12 @@ static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int 
   return dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
     if (variant == 4 || variant == 5) {
-      // segments of dimension 0 enough for about 2^18 threads
+      // segments of dimension 0 for about 2^16 (FP64 links) or 2^17 (FP32)
+      // walking threads: longer walks re-read more links from L2, fewer
+      // threads hide less latency (measured at 24^4, 32^4 and 48^4)
       const long long Vs = f->g.V / f->g.L[0];
-      const int nseg = (int)std::min<long long>(f->g.L[0], std::max<long long>(1, ((1LL << 18) + Vs - 1) / Vs));
+      const long long target = sizeof(R) == 8 ? (1LL << 16) : (1LL << 17);
+      const int nseg = (int)std::min<long long>(f->g.L[0], std::max<long long>(1, (target + Vs - 1) / Vs));
       const int S = (f->g.L[0] + nseg - 1) / nseg;
       const int nsegs = (f->g.L[0] + S - 1) / S;
       const unsigned nb = (unsigned)(nsegs * ((Vs + 255) / 256));

@@ -102,11 +102,11 @@ def test_lattice_beyond_65536_reduction_blocks():
 @pytest.mark.parametrize("variant", [4, 5])
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 def test_walking_plaquette_with_a_partial_segment(monkeypatch, storage, variant):
-    """The dimension-0 walking plaquette (variants 4, 5) on 26 x 32^3: 32,768
-    walkers per segment, 4 sites per walker, the last of 7 segments holding 2.
-    Per-site values against the oracle (FP64 1e-12, FP32 links 5e-5 per site)
-    and the sum."""
-    dims = (26, 32, 32, 32)
+    """The dimension-0 walking plaquette (variants 4, 5) on 26 x 24^3: 13,824
+    walkers per segment; FP64 links take 6 sites per walker in 5 segments,
+    FP32 links 3 in 9, the last segment holding 2 in both.  Per-site values
+    against the oracle (FP64 1e-12, FP32 links 5e-5 per site) and the sum."""
+    dims = (26, 24, 24, 24)
     U = orc.random_algebra(4, 3, dims, 0.5, 808)
     monkeypatch.setenv("KLFT_VARIANT", f"plaquette={variant}")
     g = klft.GaugeField(4, 3, dims, storage).upload(U)
