@@ -741,13 +741,26 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
     }
     x[nu] = k;
   }
+  // A device index is a sum over dimensions of the coordinate's sublattice
+  // bit (times Vh) and its encoded half-coordinate, so the shifted sites'
+  // indices below replace only the shifted coordinates' terms of x's.
+  auto dterm = [&](int d, int v) -> long long {
+    return (long long)((v & 1) << (ND - 1 - d)) * g.Vh + enc_dim(g, d, v >> 1);
+  };
+  auto wrap = [](int v, int n) { return v < n ? v : v % n; };  // v >= 0
+  const int Lmu = g.L[mu];
+  long long base = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) base += dterm(d, x[d]);
+  const long long base_nomu = base - dterm(mu, x[mu]);  // x without its mu term
+  const long long term_nu = dterm(nu, x[nu]);
   Mat<R, NC> F = identity<R, NC>();
   int Lcur = 0;
   for (int li = 0; li < nL; ++li) {
     const int L = Ls[li];
     if (valid) {
       while (Lcur < L) {
-        F = mul(F, f.load(idx_shift_k<ND>(g, x, mu, Lcur, -1, 0), mu));
+        F = mul(F, f.load(base_nomu + dterm(mu, wrap(x[mu] + Lcur, Lmu)), mu));  // U_mu(x + Lcur mu)
         ++Lcur;
       }
     }
@@ -755,16 +768,18 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
     lds[tid] = F;
     __syncthreads();
     Mat<R, NC> D = F;
+    // index of x + L mu without its nu term
+    const long long base_L = base_nomu + dterm(mu, wrap(x[mu] + L, Lmu)) - term_nu;
     for (int T = 0; T <= Tmax; ++T) {
       double w = 0.0;
       if (valid) {
         if (T > 0) {
-          const Mat<R, NC> a = f.load(idx_shift_k<ND>(g, x, nu, T - 1, -1, 0), nu);
-          const Mat<R, NC> b = f.load(idx_shift_k<ND>(g, x, mu, L, nu, T - 1), nu);
+          const long long tnu = dterm(nu, wrap(k + T - 1, Ln));
+          const Mat<R, NC> a = f.load(base - term_nu + tnu, nu);  // U_nu(x + (T-1) nu)
+          const Mat<R, NC> b = f.load(base_L + tnu, nu);          // U_nu(x + L mu + (T-1) nu)
           D = mul(mul_da(a, D), b);
         }
-        int kt = k + T;
-        kt %= Ln;
+        const int kt = wrap(k + T, Ln);
         w = (double)retr_mul_ad(D, lds[cl * Ln + kt]);
         if (per_site && li == ps_li && T == ps_T && plane == 0) per_site[lex_index<ND>(g, x)] = w;
       }

@@ -421,20 +421,6 @@ __device__ __forceinline__ long long idx_shift(const Geom& g, const int* x, int 
   return dev_index<ND>(g, v);
 }
 // the same for arbitrary non-negative shifts k1, k2
-template <int ND>
-__device__ __forceinline__ long long idx_shift_k(const Geom& g, const int* x, int m1, int k1, int m2,
-                                                 int k2) {
-  int v[ND];
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    int w = x[d];
-    if (d == m1) w += k1;
-    if (d == m2) w += k2;
-    v[d] = w % g.L[d];
-  }
-  return dev_index<ND>(g, v);
-}
-
 // Device-index offsets of a site's +-1 neighbours.  x +- e_d lies in the
 // sublattice with bit d flipped (+-Vh << (nd-1-d)) and at half-coordinate
 // y_d or y_d +- 1, so every neighbour of x -- and x + e_mu - e_nu -- is s

@@ -2,8 +2,9 @@
 
 The site order inside a sublattice is fixed when the library first builds a
 lattice geometry, so each case runs in a child process with the variable set.
-The child uploads an oracle field, runs the plaquette, one Metropolis sweep
-(every sweep variant) and one HMC force, and writes the results; the parent
+The child uploads an oracle field, runs the plaquette, three Wilson loops,
+one Metropolis sweep (every sweep variant) and one HMC force, and writes the
+results; the parent
 compares them with the oracle at the FP64 bars of test_gpu_parity.py.
 """
 import json
@@ -32,13 +33,14 @@ os.environ["KLFT_VARIANT"] = "metropolis=" + variant
 U = np.load(path + ".U.npy")
 g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
 plaq = klft.GaugePlaquette(g)
+wl = [w for (_, _, _, _, w) in klft.WilsonLoop_mu_nu(g, 0, 1, [(1, 1), (2, 3), (3, 5)])]
 p = klft.MetropolisParams(nHits=5, beta=5.5 if nc == 3 else 2.2, delta=0.3, seed=31)
 acc, n = klft.sweep_Metropolis(g, p, 0, with_count=True)
 h = klft.HMC(g, beta=5.5, tau=1.0, nsteps=4, seed=5)
 F = h.force()
 np.save(path + ".Uout.npy", g.download())
 np.save(path + ".F.npy", F)
-json.dump({"plaq": plaq, "n": n}, open(path + ".json", "w"))
+json.dump({"plaq": plaq, "n": n, "wl": wl}, open(path + ".json", "w"))
 """
 
 
@@ -54,6 +56,9 @@ def test_tiled_site_order_matches_oracle(tmp_path, nd, nc, dims, variant):
     assert r.returncode == 0, r.stdout + r.stderr
     got = json.load(open(base + ".json"))
     assert abs(got["plaq"] - orc.plaquette(U, nd, nc, dims)) < 1e-12
+    ref = orc.wilson_mu_nu(U, nd, nc, dims, 0, 1, [(1, 1), (2, 3), (3, 5)])
+    for w, r in zip(got["wl"], ref):
+        assert abs(w - r) < 1e-12
     Uo = U.copy()
     beta = 5.5 if nc == 3 else 2.2
     _, n_o = orc.metropolis_sweep(Uo, nd, nc, dims, 5, beta, 0.3, 31, 0)
