@@ -712,8 +712,9 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
 //   w(x;L,T) = Re Tr[D(L,T) F_L(x+T nu)^+],
 // which equals the reference's Re Tr[A B^+] with A = l_mu(x;L) l_nu(x+L mu;T),
 // B = l_nu(x;T) l_mu(x+T nu;L) (WilsonLoop.hpp:129-160).  No line field is
-// materialised in HBM.  Block partial sums go to
-// partials[((plane * nL + li) * (Tmax+1) + T) * gridDim.x + blockIdx.x].
+// materialised in HBM.  Wave partial sums go to
+// partials[(((plane * nL + li) * (Tmax+1) + T) * gridDim.x + blockIdx.x)
+//          * waves per block + wave].
 template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, int mu0, int nu,
                                                  int ncolblk, int nL, const int *__restrict__ Ls,
@@ -721,7 +722,6 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
                                                  int ps_li, int ps_T) {
   extern __shared__ __align__(16) unsigned char smem[];
   Mat<R, NC> *lds = reinterpret_cast<Mat<R, NC> *>(smem);
-  __shared__ double red[8];
   const int plane = blockIdx.y;
   const int mu = mu0 < 0 ? plane : mu0;
   const int Ln = g.L[nu];
@@ -783,9 +783,12 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
         w = (double)retr_mul_ad(D, lds[cl * Ln + kt]);
         if (per_site && li == ps_li && T == ps_T && plane == 0) per_site[lex_index<ND>(g, x)] = w;
       }
-      const double bt = block_sum<512>(w, red);
-      if (tid == 0)
-        partials[((long long)(plane * nL + li) * (Tmax + 1) + T) * gridDim.x + blockIdx.x] = bt;
+      // one partial per wave (no block barrier per (L,T)); the partials of
+      // an output are summed in a fixed order afterwards
+      const double ws = wave_sum(w);
+      if ((tid & 63) == 0)
+        partials[(((long long)(plane * nL + li) * (Tmax + 1) + T) * gridDim.x + blockIdx.x) * (blockDim.x >> 6) +
+                 (tid >> 6)] = ws;
     }
   }
 }
@@ -831,7 +834,8 @@ int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, 
   const long long nout = (long long)nplanes * nL * (Tmax + 1);
   double *dpart = nullptr, *dout = nullptr, *dps = nullptr;
   int *dLs = nullptr;
-  KLFT_HIP(hipMalloc(&dpart, sizeof(double) * nout * nbx));
+  const int nw = block / 64;  // one partial per wave
+  KLFT_HIP(hipMalloc(&dpart, sizeof(double) * nout * nbx * nw));
   KLFT_HIP(hipMalloc(&dout, sizeof(double) * nout));
   KLFT_HIP(hipMalloc(&dLs, sizeof(int) * nL));
   if (per_site_host) KLFT_HIP(hipMalloc(&dps, sizeof(double) * f->g.V));
@@ -852,7 +856,8 @@ int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, 
     return KLFT_OK;
   });
   if (!rc) {
-    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nout), dim3(256), 0, ki::stream(), dpart, (int)nbx, dout);
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nout), dim3(256), 0, ki::stream(), dpart, (int)nbx * nw,
+                       dout);
     hipError_t e = hipGetLastError();
     std::vector<double> all(nout);
     if (e == hipSuccess) e = hipMemcpyAsync(all.data(), dout, sizeof(double) * nout, hipMemcpyDeviceToHost, ki::stream());
