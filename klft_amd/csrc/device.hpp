@@ -939,15 +939,6 @@ __device__ __forceinline__ Mat<R, 2> exp_iq(const R* v, Mat<R, 2>*) {
 // The series is summed until the next term is below the working precision.
 template <typename R>
 __device__ __forceinline__ Mat<R, 3> exp_iq(const R* v, Mat<R, 3>*) {
-  // 1/n for n = 0..63 (entry 0 unused), as the compiler rounds the literals
-  constexpr double kInvN[64] = {
-      0.0,      1.0 / 1,  1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,  1.0 / 6,  1.0 / 7,  1.0 / 8,  1.0 / 9,
-      1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17, 1.0 / 18, 1.0 / 19,
-      1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23, 1.0 / 24, 1.0 / 25, 1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
-      1.0 / 30, 1.0 / 31, 1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35, 1.0 / 36, 1.0 / 37, 1.0 / 38, 1.0 / 39,
-      1.0 / 40, 1.0 / 41, 1.0 / 42, 1.0 / 43, 1.0 / 44, 1.0 / 45, 1.0 / 46, 1.0 / 47, 1.0 / 48, 1.0 / 49,
-      1.0 / 50, 1.0 / 51, 1.0 / 52, 1.0 / 53, 1.0 / 54, 1.0 / 55, 1.0 / 56, 1.0 / 57, 1.0 / 58, 1.0 / 59,
-      1.0 / 60, 1.0 / 61, 1.0 / 62, 1.0 / 63};
   const R r3 = (R)0.57735026918962576451;
   Mat<R, 3> q;
   q.m[0][0] = mk(v[2] + v[7] * r3, R(0));
@@ -970,9 +961,7 @@ __device__ __forceinline__ Mat<R, 3> exp_iq(const R* v, Mat<R, 3>*) {
   // spectral radius bound: sum of squared eigenvalues is 2 c1
   const R rho = sqrt(R(2) * c1);
   for (int n = 1; n < 64; ++n) {
-    // 1/n without a division: the correctly rounded double (exactly what
-    // R(1) / n gives in FP64), rounded once more to float for FP32
-    const R inv = (R)kInvN[n];
+    const R inv = R(1) / (R)n;
     const R an = c0 * c * inv, bn = (a + c1 * c) * inv, cn = b * inv;
     a = an;
     b = bn;
