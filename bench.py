@@ -215,7 +215,7 @@ def _hmc_chain(klft, st, nsteps, ntraj, therm):
     """One SU(2) 16^4 HMC chain (beta 2.3, tau 1, Haar hot start): rates,
     acceptance, binned plaquette and <exp(-dH)> after `therm` trajectories."""
     import numpy as np
-    g = klft.GaugeField(4, 2, (16, 16, 16, 16), st).set_haar(2 * 1234)
+    g = klft.GaugeField(4, 2, (16, 16, 16, 16), st).set_random_sun(0.5, 2 * 1234)
     h = klft.HMC(g, beta=2.3, tau=1.0, nsteps=nsteps, seed=1234)
     dH, acc, plaq = [], [], []
     t0 = time.perf_counter()

@@ -69,6 +69,14 @@ int klft_field_set_random_algebra(klft_field_t f, double eps, uint64_t seed);
 /* Haar-random links (SU(2): normalised Gaussian quaternion; U(1): uniform
  * phase; SU(3): Gram-Schmidt of a Gaussian matrix, third row conj(r0 x r1)). */
 int klft_field_set_haar(klft_field_t f, uint64_t seed);
+/* deviceGaugeField(L.., rng) (GaugeField.hpp:58-63, :118-141): every entry
+ * of every link an independent complex(drand(-1,1), drand(-1,1)), real part
+ * first, c1 outer, c2 inner, from the keyed stream (tag 6).  The links are
+ * general complex matrices, so two-row storage (Nc > 1) is rejected. */
+int klft_field_set_random_complex(klft_field_t f, uint64_t seed);
+/* deviceGaugeField(L.., complex_t init) (GaugeField.hpp:36-40, :65-84):
+ * every entry of every link = re + i im; two-row storage (Nc > 1) rejected. */
+int klft_field_set_constant(klft_field_t f, double re, double im);
 /* host AoS FP64 <-> device (rounded to the storage precision on upload) */
 int klft_field_upload(klft_field_t f, const double *host);
 int klft_field_download(klft_field_t f, double *host);

@@ -5,8 +5,9 @@ by ``python -m klft_amd.build``).  Names mirror the reference's C++ API
 (deviceGaugeField, MetropolisParams, sweep_Metropolis, GaugePlaquette,
 WilsonLoop_mu_nu, WilsonLoop_temporal, HMC ...), with the same argument
 meaning and error behaviour: a failing call raises KlftError carrying the
-library's message.  There is no CPU fallback: importing the package fails if
-the HIP library has not been built.
+library's message.  There is no CPU fallback: the library is loaded on first
+use (so ``python -m klft_amd.build`` runs on a clean checkout), and every call
+raises ImportError if it has not been built.
 """
 from __future__ import annotations
 
@@ -38,7 +39,35 @@ def _load() -> C.CDLL:
     return C.CDLL(str(LIB_PATH), mode=C.RTLD_GLOBAL)
 
 
-lib = _load()
+class _Library:
+    """libklft_hip.so, opened and given its ctypes signatures on first use."""
+
+    def __init__(self):
+        self._dll = None
+        self.exported: list[str] = []
+
+    def load(self) -> C.CDLL:
+        if self._dll is None:
+            dll = _load()
+            names = []
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(dll, name, None)
+                if fn is None:
+                    continue
+                fn.restype = res
+                fn.argtypes = args
+                names.append(name)
+            self.exported[:] = names
+            self._dll = dll
+        return self._dll
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        return getattr(self.load(), name)
+
+
+lib = _Library()
 
 _p = C.c_void_p
 _dp = C.POINTER(C.c_double)
@@ -70,6 +99,8 @@ _SIGS = {
     "klft_field_set_random_sun": (C.c_int, [_p, C.c_double, C.c_uint64]),
     "klft_field_set_random_algebra": (C.c_int, [_p, C.c_double, C.c_uint64]),
     "klft_field_set_haar": (C.c_int, [_p, C.c_uint64]),
+    "klft_field_set_random_complex": (C.c_int, [_p, C.c_uint64]),
+    "klft_field_set_constant": (C.c_int, [_p, C.c_double, C.c_double]),
     "klft_field_upload": (C.c_int, [_p, _dp]),
     "klft_field_download": (C.c_int, [_p, _dp]),
     "klft_field_copy": (C.c_int, [_p, _p]),
@@ -115,14 +146,13 @@ _SIGS = {
     "klft_profile_read": (C.c_int, [C.c_char_p, _dp, _i64p]),
 }
 
-EXPORTED = []
-for _name, (_res, _args) in _SIGS.items():
-    fn = getattr(lib, _name, None)
-    if fn is None:
-        continue
-    fn.restype = _res
-    fn.argtypes = _args
-    EXPORTED.append(_name)
+# names of _SIGS the loaded library exports (filled by the first load)
+EXPORTED = lib.exported
+
+
+def load() -> C.CDLL:
+    """Open libklft_hip.so now (ImportError if it has not been built)."""
+    return lib.load()
 
 
 def _check(rc: int) -> None:
@@ -195,6 +225,17 @@ class GaugeField:
 
     def set_haar(self, seed: int):
         _check(lib.klft_field_set_haar(self.h, seed))
+        return self
+
+    def set_random_complex(self, seed: int):
+        """deviceGaugeField(L.., rng) (GaugeField.hpp:118-141)."""
+        _check(lib.klft_field_set_random_complex(self.h, seed))
+        return self
+
+    def set_constant(self, value: complex):
+        """deviceGaugeField(L.., complex_t init) (GaugeField.hpp:65-84)."""
+        value = complex(value)
+        _check(lib.klft_field_set_constant(self.h, value.real, value.imag))
         return self
 
     def upload(self, U: np.ndarray):

@@ -175,9 +175,11 @@ __global__ void k_defect_sums(const ChainDev *__restrict__ chains, int n, Defect
   Links<R, NC, COMP> f{reinterpret_cast<cx<R> *>(ch.links), g.V, 4};
   double a1 = 0.0, a2 = 0.0;
   for (int nu = 1; nu < 4; ++nu) {
-    // base sites y: y0 = 0, y_nu in [p_nu - 1, p_nu + Ld), the others in [p_k, p_k + Ld)
+    // base sites y: y0 = 0, y_nu in [p_nu - 1, p_nu + Ld), the others in
+    // [p_k, p_k + Ld); when Ld equals L_nu, p_nu - 1 and p_nu + Ld - 1 are
+    // the same site, so the range along nu is cut to L_nu sites
     int ext[4] = {1, d.len, d.len, d.len};
-    ext[nu] = d.len + 1;
+    ext[nu] = d.len + 1 < g.L[nu] ? d.len + 1 : g.L[nu];
     for (int i1 = 0; i1 < ext[1]; ++i1)
       for (int i2 = 0; i2 < ext[2]; ++i2)
         for (int i3 = 0; i3 < ext[3]; ++i3) {
@@ -222,6 +224,17 @@ __global__ void k_swap(int n, int nc, double beta, const double *__restrict__ cv
     }
   }
   for (int k = 0; k < n; ++k) cval[conf_of[k]] = cvals[k];
+}
+
+// A1 / A2 of every replica into A (one thread per replica)
+template <typename R, bool COMP>
+void launch_defect_sums(int nc, const ChainDev *chains, int n, const Defect &d, double *A) {
+  const dim3 grid((unsigned)((n + 63) / 64)), block(64);
+  switch (nc) {
+    case 1: hipLaunchKernelGGL((k_defect_sums<R, 1, false>), grid, block, 0, ki::stream(), chains, n, d, A); break;
+    case 2: hipLaunchKernelGGL((k_defect_sums<R, 2, COMP>), grid, block, 0, ki::stream(), chains, n, d, A); break;
+    default: hipLaunchKernelGGL((k_defect_sums<R, 3, COMP>), grid, block, 0, ki::stream(), chains, n, d, A); break;
+  }
 }
 
 __global__ void k_zero_counters(unsigned long long *p, int n) {
@@ -336,11 +349,7 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
                            b->defect, b->d_cval);
     }
     if (b->ptbc) {
-      switch (b->nc) {
-        case 1: hipLaunchKernelGGL((k_defect_sums<R, 1, false>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
-        case 2: hipLaunchKernelGGL((k_defect_sums<R, 2, COMP>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
-        default: hipLaunchKernelGGL((k_defect_sums<R, 3, COMP>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
-      }
+      launch_defect_sums<R, COMP>(b->nc, b->d_chains, b->n, b->defect, b->d_A);
       hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, ki::stream(), b->n, b->nc, b->beta[0], b->d_cvals, b->d_A,
                          b->d_conf_of, b->d_cval, b->d_swaps, seed, b->d_sweep);
     }
@@ -584,11 +593,7 @@ extern "C" int klft_ptbc_action(klft_batch_t b, int replica, int coupling, doubl
   auto launch = [&](auto rt, auto ct) -> int {
     using R = typename decltype(rt)::type;
     constexpr bool COMP = decltype(ct)::value != 0;
-    switch (b->nc) {
-      case 1: hipLaunchKernelGGL((k_defect_sums<R, 1, false>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
-      case 2: hipLaunchKernelGGL((k_defect_sums<R, 2, COMP>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
-      default: hipLaunchKernelGGL((k_defect_sums<R, 3, COMP>), dim3(1), dim3(64), 0, ki::stream(), b->d_chains, b->n, b->defect, b->d_A); break;
-    }
+    launch_defect_sums<R, COMP>(b->nc, b->d_chains, b->n, b->defect, b->d_A);
     KLFT_HIP(hipGetLastError());
     return KLFT_OK;
   };

@@ -178,10 +178,12 @@ __global__ __launch_bounds__(256) void k_download(Links<R, NC, COMP> f, Geom g,
   }
 }
 
-// kind: 0 identity, 1 randSUN(delta), 2 exp(i eps Q), 3 Haar
+// kind: 0 identity, 1 randSUN(delta), 2 exp(i eps Q), 3 Haar, 4 every entry
+// an independent complex (drand(-1,1), drand(-1,1)), 5 every entry the
+// constant (param, param2)
 template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(256) void k_init(Links<R, NC, COMP> f, Geom g, int kind,
-                                               double param, uint64_t seed) {
+                                               double param, double param2, uint64_t seed) {
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;  // lexicographic
   if (s >= g.V) return;
   int x[ND];
@@ -193,8 +195,25 @@ __global__ __launch_bounds__(256) void k_init(Links<R, NC, COMP> f, Geom g, int 
     if (kind == 0) {
       u = identity<double, NC>();
     } else if (kind == 1) {
-      klft_rng::Stream st(seed, 5, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
+      klft_rng::Stream st(seed, klft_rng::TAG_RANDSUN, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
       rand_sun(st, param, u);
+    } else if (kind == 4) {
+      // GaugeField.hpp:118-141: real part, then imaginary part, c1 outer, c2 inner
+      klft_rng::Stream st(seed, klft_rng::TAG_COMPLEX, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+          const double re = draw(st, -1.0, 1.0);
+          const double im = draw(st, -1.0, 1.0);
+          u.m[i][j] = mk(re, im);
+        }
+    } else if (kind == 5) {
+      // GaugeField.hpp:65-84
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) u.m[i][j] = mk(param, param2);
     } else if (kind == 2) {
       klft_rng::Stream st(seed, klft_rng::TAG_ALGEBRA, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
       double h[NG + 1];
@@ -342,6 +361,7 @@ extern "C" int klft_field_destroy(klft_field_t f) {
   if (f->links) (void)hipFree(f->links);
   if (f->partials) (void)hipFree(f->partials);
   if (f->host_result) (void)hipHostFree(f->host_result);
+  if (f->run_counters) (void)hipFree(f->run_counters);
   f->links = nullptr;
   delete f;
   return KLFT_OK;
@@ -358,13 +378,13 @@ extern "C" int klft_field_info(klft_field_t f, int *nd, int *nc, int *dims, int 
   return KLFT_OK;
 }
 
-static int launch_init(klft_field_t f, int kind, double param, uint64_t seed) {
+static int launch_init(klft_field_t f, int kind, double param, uint64_t seed, double param2 = 0.0) {
   int rc = ki::check_field(f);
   if (rc) return rc;
   rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
     hipLaunchKernelGGL((k_init<R, NC, COMP, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256), 0,
-                       ki::stream(), ki::links_of<R, NC, COMP>(f), f->g, kind, param, seed);
+                       ki::stream(), ki::links_of<R, NC, COMP>(f), f->g, kind, param, param2, seed);
     KLFT_HIP(hipGetLastError());
     return KLFT_OK;
   });
@@ -380,6 +400,24 @@ extern "C" int klft_field_set_random_algebra(klft_field_t f, double eps, uint64_
   return launch_init(f, 2, eps, seed);
 }
 extern "C" int klft_field_set_haar(klft_field_t f, uint64_t seed) { return launch_init(f, 3, 0.0, seed); }
+
+// general complex matrices: the two-row storage keeps SU(N) links only (the
+// last row is rebuilt from the others), so it cannot hold them
+static int check_general_matrices(klft_field_t f) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (f->storage == KLFT_STORAGE_F32_ROWS && f->nc > 1)
+    return ki::set_error(KLFT_E_ARG, "two-row storage holds SU(N) links only; use F64 or F32 storage");
+  return KLFT_OK;
+}
+extern "C" int klft_field_set_random_complex(klft_field_t f, uint64_t seed) {
+  int rc = check_general_matrices(f);
+  return rc ? rc : launch_init(f, 4, 0.0, seed);
+}
+extern "C" int klft_field_set_constant(klft_field_t f, double re, double im) {
+  int rc = check_general_matrices(f);
+  return rc ? rc : launch_init(f, 5, re, 0, im);
+}
 
 extern "C" int klft_field_upload(klft_field_t f, const double *host) {
   int rc = ki::check_field(f);

@@ -27,6 +27,7 @@ struct klft_hmc_s {
   void *P = nullptr;       // momenta (R)
   void *Ubak = nullptr;    // saved links for a rejected trajectory
   double *Fdbg = nullptr;  // force in host (AoS) order, for klft_hmc_force
+  double *stage = nullptr; // host-order FP64 momenta for upload / download
 };
 
 template <typename R, int NC, int ND>
@@ -338,8 +339,8 @@ int enqueue_leapfrog(klft_hmc_t h, int kin_slot) {
 int momenta_io(klft_hmc_t h, double *host, int upload) {
   klft_field_t f = h->f;
   const size_t n = (size_t)f->g.V * f->nd * ngen(f->nc) * sizeof(double);
-  double *stage = nullptr;
-  KLFT_HIP(hipMalloc(&stage, n));
+  if (!h->stage) KLFT_HIP(hipMalloc(&h->stage, n));  // kept with the handle
+  double *stage = h->stage;
   hipError_t e = hipSuccess;
   if (upload) e = hipMemcpyAsync(stage, host, n, hipMemcpyHostToDevice, ki::stream());
   int rc = KLFT_OK;
@@ -354,7 +355,6 @@ int momenta_io(klft_hmc_t h, double *host, int upload) {
     });
   if (e == hipSuccess && !rc && !upload) e = hipMemcpyAsync(host, stage, n, hipMemcpyDeviceToHost, ki::stream());
   if (e == hipSuccess) e = hipStreamSynchronize(ki::stream());
-  (void)hipFree(stage);
   if (e != hipSuccess) return ki::hip_fail(e, "momenta transfer");
   return rc;
 }
@@ -392,6 +392,7 @@ extern "C" int klft_hmc_destroy(klft_hmc_t h) {
   if (h->P) (void)hipFree(h->P);
   if (h->Ubak) (void)hipFree(h->Ubak);
   if (h->Fdbg) (void)hipFree(h->Fdbg);
+  if (h->stage) (void)hipFree(h->stage);
   h->P = nullptr;
   delete h;
   return KLFT_OK;

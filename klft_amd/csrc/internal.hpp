@@ -28,6 +28,10 @@ struct klft_field_s {
   double *result = nullptr;  // 16 doubles
   unsigned long long *counter = nullptr;  // 8 counters
   double *host_result = nullptr;          // pinned, 16 doubles
+  // per-sweep acceptance counters of klft_metropolis_run (grown on demand,
+  // kept until the field is destroyed)
+  unsigned long long *run_counters = nullptr;
+  int run_capacity = 0;
 };
 
 namespace klft_internal {
@@ -86,6 +90,8 @@ bool tuning_enabled();
 std::string tune_key(const char *kernel, const klft_field_s *f);
 int pick_variant(const char *kernel, const klft_field_s *f, int nvar, int dflt,
                  const std::function<int(int)> &run);
+// the forced, cached or default variant, never timed
+int chosen_variant(const char *kernel, const klft_field_s *f, int nvar, int dflt);
 // the variant KLFT_VARIANT forces for `kernel` ("name=v,..."), or -1
 int forced_variant(const char *kernel);
 

@@ -14,6 +14,7 @@
 // (M r and S restoreSUN(U r) differ by rounding only).
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "internal.hpp"
@@ -34,7 +35,8 @@ int launch_passes_f32rows(klft_field_t f, void *links, int variant, int p0, int 
 }  // namespace klft_internal
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+  for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int)(gridDim.x * blockDim.x))
+    p[i] = 0ull;
 }
 
 namespace klft_internal {
@@ -60,7 +62,7 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
 
 // sweep variant for this field; tuning times one pass of each variant on a
 // scratch copy of the links, so the live field is never updated by it
-int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
+int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool timed) {
   void *scratch = nullptr;
   // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4
   // waves/SIMD; 4 one lane with the low-register staple at >= 2 waves/SIMD;
@@ -76,7 +78,10 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
   // order for FP64 links and the upper-terms-first order for FP32 links
   // (the interleaved order loses at 48^4 in FP32).
   const int dflt = f->storage == KLFT_STORAGE_F64 ? 13 : 12;
-  const int v = pick_variant("metropolis", f, kMetroVariants, dflt, [&](int var) -> int {
+  // the tuning key carries nHits: the best variant depends on the hit count
+  const std::string kname = "metropolis_nhits_" + std::to_string(p->nhits);
+  if (!timed) return chosen_variant(kname.c_str(), f, kMetroVariants, dflt);
+  const int v = pick_variant(kname.c_str(), f, kMetroVariants, dflt, [&](int var) -> int {
     if (!scratch) {
       KLFT_HIP(hipMalloc(&scratch, f->link_bytes));
       KLFT_HIP(hipMemcpyAsync(scratch, f->links, f->link_bytes, hipMemcpyDeviceToDevice, stream()));
@@ -93,7 +98,9 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p) {
 // enqueue one sweep (no synchronisation); acceptance counted into *counter
 int enqueue_sweep(klft_field_t f, const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
                   unsigned long long *counter, int *visit) {
-  const int v = metropolis_variant(f, p);
+  // the schedule dump (visit != nullptr, nHits = 0) is never timed: its
+  // staple-only passes are not the workload a tuned variant is chosen for
+  const int v = metropolis_variant(f, p, visit == nullptr);
   return launch_passes(f, f->links, v, 0, 1 << f->nd, p, sweep, chain, counter, visit);
 }
 
@@ -138,8 +145,21 @@ extern "C" int klft_metropolis_run(klft_field_t f, const klft_metropolis_params 
   if (n < 0) return ki::set_error(KLFT_E_ARG, "negative sweep count");
   unsigned long long *cnt = nullptr;
   if (acc_rates && n > 0) {
-    KLFT_HIP(hipMalloc(&cnt, sizeof(unsigned long long) * n));
-    KLFT_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * n, ki::stream()));
+    // one counter per sweep, in a buffer kept with the field (grown only when
+    // a longer run asks for more) and cleared by a kernel
+    if (n > f->run_capacity) {
+      if (f->run_counters) {
+        KLFT_HIP(hipStreamSynchronize(ki::stream()));
+        (void)hipFree(f->run_counters);
+        f->run_counters = nullptr;
+        f->run_capacity = 0;
+      }
+      const int cap = n > 1024 ? n : 1024;
+      KLFT_HIP(hipMalloc(&f->run_counters, sizeof(unsigned long long) * cap));
+      f->run_capacity = cap;
+    }
+    cnt = f->run_counters;
+    hipLaunchKernelGGL(k_zero_u64, dim3(ki::blocks_for(n, 256)), dim3(256), 0, ki::stream(), cnt, n);
   } else {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ki::stream(), f->counter, 1);
   }
@@ -152,7 +172,6 @@ extern "C" int klft_metropolis_run(klft_field_t f, const klft_metropolis_params 
     if (e != hipSuccess) rc = ki::hip_fail(e, "acceptance counters");
     for (int i = 0; i < n && !rc; ++i) acc_rates[i] = acceptance(f, p->nhits, h[i]);
   }
-  if (cnt) (void)hipFree(cnt);
   if (rc) return rc;
   return ki::sync_stream();
 }
@@ -161,7 +180,7 @@ extern "C" int klft_metropolis_schedule(klft_field_t f, int32_t *visit) {
   int rc = ki::check_field(f);
   if (rc) return rc;
   if (!visit) return ki::set_error(KLFT_E_ARG, "null visit buffer");
-  int *dv = nullptr;
+  int *dv = nullptr;  // diagnostic dump, not a hot path
   KLFT_HIP(hipMalloc(&dv, sizeof(int) * f->g.V));
   KLFT_HIP(hipMemsetAsync(dv, 0xff, sizeof(int) * f->g.V, ki::stream()));
   // nhits = 0: the real pass kernels run their thread-to-site map and record

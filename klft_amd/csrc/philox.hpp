@@ -11,6 +11,8 @@
 //   tag 2  HMC momenta         c0 = site, c1 = trajectory
 //   tag 3  random-algebra start c0 = site
 //   tag 4  Haar start          c0 = site
+//   tag 5  randSUN(delta) start c0 = site
+//   tag 6  random-complex start c0 = site
 //   tag 7  HMC accept          c0 = 0, c1 = trajectory
 //   tag 8  replica swap        c0 = replica pair, c1 = step
 // The reference (Metropolis.hpp:34, :76) draws from Kokkos'
@@ -32,6 +34,8 @@ enum : uint32_t {
   TAG_MOMENTA = 2,
   TAG_ALGEBRA = 3,
   TAG_HAAR = 4,
+  TAG_RANDSUN = 5,
+  TAG_COMPLEX = 6,
   TAG_ACCEPT = 7,
   TAG_SWAP = 8
 };

@@ -1,9 +1,11 @@
 // `hmc` driver (hmc.cpp:19-73, lib/HMC_SU2.cpp:8-75): same flags and defaults;
 // --gauge-group accepts SU2, U1 and (added) SU3, --precision f64|f32|f32_rows
-// selects the link storage.  Hot start: Haar-random links keyed by seed*2
-// (the reference's set_random(0.5, RNG(seed*2)) belongs to a class that is
-// not in its sources).  Output: "traj, accept, plaquette, time, acceptance
-// rate" with one line per trajectory.
+// selects the link storage.  Hot start: the reference's
+// gauge_field.set_random(0.5, RNG(seed*2)) (lib/HMC_SU2.cpp:41-42), read as
+// every link a randSUN(0.5) draw (the deviceGaugeField(L.., rng, delta)
+// start, GaugeField.hpp:96-112) from the stream keyed by seed*2.  Output:
+// "traj, accept, plaquette, time, acceptance rate" with one line per
+// trajectory.
 #include <chrono>
 #include <fstream>
 #include <iostream>
@@ -69,7 +71,7 @@ int main(int argc, char **argv) {
   klft_hmc_t h = nullptr;
   try {
     klft::check(klft_field_create(ndim, nc, dims, storage, &f));
-    klft::check(klft_field_set_haar(f, (uint64_t)seed * 2));
+    klft::check(klft_field_set_random_sun(f, 0.5, (uint64_t)seed * 2));
     klft_hmc_params p{beta, tau, (int)n_steps, (uint64_t)seed};
     klft::check(klft_hmc_create(f, &p, &h));
     real_t plaq = 0.0;

@@ -52,20 +52,46 @@ std::string tune_key(const char *kernel, const klft_field_s *f) {
   return k;
 }
 
+// KLFT_VARIANT is parsed once per distinct value (the launchers ask on every
+// call, and the value may change between calls)
+static std::mutex g_vmu;
+static std::string g_venv;
+static bool g_venv_set = false;
+static std::map<std::string, int> g_forced;
+
 int forced_variant(const char *kernel) {
   const char *e = std::getenv("KLFT_VARIANT");
-  if (!e) return -1;
-  const std::string s(e), name(kernel);
-  size_t pos = 0;
-  while (pos < s.size()) {
-    size_t end = s.find(',', pos);
-    if (end == std::string::npos) end = s.size();
-    const std::string item = s.substr(pos, end - pos);
-    const size_t eq = item.find('=');
-    if (eq != std::string::npos && item.substr(0, eq) == name) return std::atoi(item.c_str() + eq + 1);
-    pos = end + 1;
+  std::lock_guard<std::mutex> lk(g_vmu);
+  if (!e) {
+    g_venv_set = false;
+    return -1;
   }
-  return -1;
+  if (!g_venv_set || g_venv != e) {
+    g_venv = e;
+    g_venv_set = true;
+    g_forced.clear();
+    size_t pos = 0;
+    while (pos < g_venv.size()) {
+      size_t end = g_venv.find(',', pos);
+      if (end == std::string::npos) end = g_venv.size();
+      const std::string item = g_venv.substr(pos, end - pos);
+      const size_t eq = item.find('=');
+      if (eq != std::string::npos) g_forced[item.substr(0, eq)] = std::atoi(item.c_str() + eq + 1);
+      pos = end + 1;
+    }
+  }
+  auto it = g_forced.find(kernel);
+  return it == g_forced.end() ? -1 : it->second;
+}
+
+int chosen_variant(const char *kernel, const klft_field_s *f, int nvar, int dflt) {
+  const int forced = forced_variant(kernel);
+  if (forced >= 0 && forced < nvar) return forced;
+  const std::string key = tune_key(kernel, f);
+  std::lock_guard<std::mutex> lk(g_tmu);
+  auto it = g_table.find(key);
+  if (it != g_table.end() && it->second.variant >= 0 && it->second.variant < nvar) return it->second.variant;
+  return dflt;
 }
 
 int pick_variant(const char *kernel, const klft_field_s *f, int nvar, int dflt,

@@ -695,6 +695,44 @@ void orc_field_random_algebra(int nd, int nc, const int *dims, double eps, uint6
     }
 }
 
+/* GaugeField.hpp:55-60, :96-112: every link a randSUN(delta) draw, from the
+ * stream (tag 5, c0 = lexicographic site, mu) */
+void orc_field_random_sun(int nd, int nc, const int *dims, double delta, uint64_t seed,
+                          double *U) {
+  lat l = mklat(nd, nc, dims);
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < l.V; ++s)
+    for (int mu = 0; mu < nd; ++mu) {
+      stream st;
+      stream_init(&st, seed, 5, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
+      sun r;
+      rand_sun(nc, &st, delta, &r);
+      set_link(&l, U, s, mu, &r);
+    }
+}
+
+/* GaugeField.hpp:58-63, :118-141: every entry complex(drand(-1,1),
+ * drand(-1,1)), real part drawn first, c1 outer, c2 inner; stream (tag 6,
+ * c0 = lexicographic site, mu) */
+void orc_field_random_complex(int nd, int nc, const int *dims, uint64_t seed, double *U) {
+  lat l = mklat(nd, nc, dims);
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < l.V; ++s)
+    for (int mu = 0; mu < nd; ++mu) {
+      stream st;
+      stream_init(&st, seed, 6, (uint32_t)s, 0, 0, 0, (uint32_t)mu);
+      sun a = zero_sun();
+      for (int c1 = 0; c1 < nc; ++c1)
+        for (int c2 = 0; c2 < nc; ++c2) {
+          double re = drand(&st, -1.0, 1.0);
+          double im = drand(&st, -1.0, 1.0);
+          a.m[c1][c2].re = re;
+          a.m[c1][c2].im = im;
+        }
+      set_link(&l, U, s, mu, &a);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Metropolis (Metropolis.hpp:72-178)                                       */
 /* ------------------------------------------------------------------------ */

@@ -46,6 +46,13 @@ void orc_field_identity(int nd, int nc, const int *dims, double *U);
 /* U = exp(i eps Q), Q = sum_a h_a T_a / 2, h_a ~ N(0,1) keyed (seed, site, mu) */
 void orc_field_random_algebra(int nd, int nc, const int *dims, double eps,
                               uint64_t seed, double *U);
+/* every link randSUN(delta) keyed (seed, site, mu) (GaugeField.hpp:96-112) */
+void orc_field_random_sun(int nd, int nc, const int *dims, double delta,
+                          uint64_t seed, double *U);
+/* every entry complex(drand(-1,1), drand(-1,1)) keyed (seed, site, mu)
+ * (GaugeField.hpp:118-141) */
+void orc_field_random_complex(int nd, int nc, const int *dims, uint64_t seed,
+                              double *U);
 
 /* --- measurement --------------------------------------------------------- */
 void orc_staple(int nd, int nc, const int *dims, const double *U, long site,

@@ -1,8 +1,5 @@
-import os
 import sys
 from pathlib import Path
-
-import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
 sys.path.insert(0, str(ROOT))
@@ -14,6 +11,9 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long ensemble test")
 
 
-def pytest_collection_modifyitems(config, items):
-    # make sure the HIP library is present for GPU tests; no silent fallback
-    pass
+def pytest_runtest_setup(item):
+    # a GPU test runs the HIP library or fails: a missing libklft_hip.so is an
+    # error, never a skip or a fallback
+    if item.get_closest_marker("gpu") is not None:
+        import klft_amd
+        klft_amd.load()

@@ -38,6 +38,8 @@ L.orc_rand_sun.restype = C.c_int
 L.orc_exp_iq.argtypes = [C.c_int, _dp, _dp]
 L.orc_field_identity.argtypes = [C.c_int, C.c_int, _ip, _dp]
 L.orc_field_random_algebra.argtypes = [C.c_int, C.c_int, _ip, C.c_double, _u64, _dp]
+L.orc_field_random_sun.argtypes = [C.c_int, C.c_int, _ip, C.c_double, _u64, _dp]
+L.orc_field_random_complex.argtypes = [C.c_int, C.c_int, _ip, _u64, _dp]
 L.orc_staple.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_long, C.c_int, _dp]
 L.orc_staple_field.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp]
 L.orc_plaquette.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, _dp]
@@ -107,6 +109,18 @@ def identity(nd, nc, dims):
 def random_algebra(nd, nc, dims, eps, seed):
     U = np.empty(field_shape(nd, nc, dims), dtype=np.complex128)
     L.orc_field_random_algebra(nd, nc, _i(dims), eps, seed, _d(U))
+    return U
+
+
+def random_sun_field(nd, nc, dims, delta, seed):
+    U = np.empty(field_shape(nd, nc, dims), dtype=np.complex128)
+    L.orc_field_random_sun(nd, nc, _i(dims), delta, seed, _d(U))
+    return U
+
+
+def random_complex_field(nd, nc, dims, seed):
+    U = np.empty(field_shape(nd, nc, dims), dtype=np.complex128)
+    L.orc_field_random_complex(nd, nc, _i(dims), seed, _d(U))
     return U
 
 

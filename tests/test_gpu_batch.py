@@ -11,7 +11,7 @@ import pytest
 import oracle as orc
 
 pytestmark = pytest.mark.gpu
-klft = pytest.importorskip("klft_amd")
+import klft_amd as klft  # noqa: E402
 
 CHAINS = [(1, (4, 6, 4, 4), 1.5), (2, (6, 4, 4, 8), 2.1), (3, (4, 4, 6, 4), 5.2), (2, (4, 4, 4, 4), 2.5),
           (1, (8, 4, 4, 6), 2.0)]
@@ -135,3 +135,43 @@ def test_ptbc_c5_shape_runs_in_fp32():
     assert sorted(p.labels()) == list(range(nrep))
     pl = p.plaquette()
     assert np.all((pl > 0.3) & (pl <= 1.0))
+
+
+@pytest.mark.parametrize("nrep,nc,dims,L,pos", [(70, 2, (4, 4, 4, 4), 2, [1, 3, 0]),
+                                               (5, 3, (4, 6, 4, 4), 4, [0, 2, 1])])
+def test_ptbc_many_replicas_and_full_length_defect(nrep, nc, dims, L, pos):
+    """More than 64 replicas (the defect sums' grid covers every replica), and
+    a defect as long as a spatial extent (L = L1 = L3 = 4: no plaquette is
+    counted twice): swap decisions, labels and every replica's action under
+    every tested coupling against the oracle's defect-scaled action."""
+    beta, seed = 2.3 if nc == 2 else 5.5, 97
+    c = np.linspace(1.0, 0.0, nrep)
+    p = klft.PTBC(nrep, nc, dims, beta, L, pos, c, "f64")
+    refs = [orc.random_algebra(4, nc, dims, 0.3, 900 + r) for r in range(nrep)]
+    for r in range(nrep):
+        p.upload(r, refs[r])
+
+    def S(U, ck):
+        return -(beta / nc) * orc.plaquette_defect(U, orc.defect_field(4, dims, L, pos, ck), 4, nc, dims,
+                                                   normalize=False)
+    for r in [0, nrep // 2, nrep - 1]:
+        for k in [0, nrep - 1]:
+            want = S(refs[r], c[k])
+            assert abs(p.action(r, k) - want) < 1e-9 * abs(want), (r, k)
+    conf_of = list(range(nrep))
+    swaps = p.step(3, 0.2, seed, 0)
+    cval = {conf_of[k]: c[k] for k in range(nrep)}
+    for r in range(nrep):
+        orc.metropolis_sweep_defect(refs[r], orc.defect_field(4, dims, L, pos, cval[r]), 4, nc, dims, 3, beta, 0.2,
+                                    seed, 0, r)
+    want = []
+    for k in range(nrep - 1):
+        a, b_ = conf_of[k], conf_of[k + 1]
+        dS = S(refs[a], c[k + 1]) + S(refs[b_], c[k]) - S(refs[a], c[k]) - S(refs[b_], c[k + 1])
+        u = orc.stream_doubles(seed, 8, k, 0, 0, 0, 0, 1)[0]
+        acc = dS <= 0 or u < np.exp(-dS)
+        want.append(int(acc))
+        if acc:
+            conf_of[k], conf_of[k + 1] = b_, a
+    assert list(swaps) == want
+    assert list(p.labels()) == conf_of

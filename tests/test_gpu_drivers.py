@@ -14,7 +14,7 @@ import pytest
 import oracle as orc
 
 pytestmark = pytest.mark.gpu
-klft = pytest.importorskip("klft_amd")
+import klft_amd as klft  # noqa: E402
 ROOT = Path(__file__).resolve().parents[1]
 BIN = ROOT / "klft_amd"
 
@@ -155,6 +155,28 @@ def test_hmc_driver_output(tmp_path):
     for i, l in enumerate(lines[1:]):
         f = l.split(", ")
         assert int(f[0]) == i and f[1] in ("0", "1") and 0 < float(f[2]) < 1
+
+
+@pytest.mark.parametrize("group,nc,beta", [("SU2", 2, 2.3), ("U1", 1, 1.0)])
+def test_hmc_driver_values_match_oracle(tmp_path, group, nc, beta):
+    """The hmc driver's per-trajectory accept and plaquette columns against
+    the oracle's trajectories from the same start: set_random(0.5,
+    RNG(seed*2)) as randSUN(0.5) keyed by 2 * seed (lib/HMC_SU2.cpp:41-42),
+    momenta and accept draws keyed by seed and trajectory.  The plaquette is
+    printed with 6 significant digits (default ostream formatting)."""
+    seed, dims, ntraj, nsteps = 1234, (4, 4, 4, 6), 5, 10
+    r = subprocess.run([str(BIN / "hmc"), "--gauge-group", group, "--LX", "4", "--LY", "4", "--LZ", "4", "--LT", "6",
+                        "--n-traj", str(ntraj), "--n-steps", str(nsteps), "--beta", str(beta), "--seed", str(seed),
+                        "--outfilename", "hmc.out"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    rows = [l.split(", ") for l in (tmp_path / "hmc.out").read_text().splitlines()[1:]]
+    U = orc.random_sun_field(4, nc, dims, 0.5, 2 * seed)
+    start = float(r.stdout.split("Starting Plaquette: ")[1].split()[0])
+    assert start == float(f"{orc.plaquette(U, 4, nc, dims):.6g}")
+    for i, row in enumerate(rows):
+        acc, _, plaq = orc.hmc_trajectory(U, 4, nc, dims, beta, 1.0, nsteps, seed, i)
+        assert int(row[0]) == i and int(row[1]) == int(acc)
+        assert float(row[2]) == float(f"{plaq:.6g}"), (i, row[2], plaq)
 
 
 @pytest.mark.parametrize("tool", ["plaq_bench", "wloop_temporal_bench", "metropolis_bench"])

@@ -14,7 +14,7 @@ import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
-klft = pytest.importorskip("klft_amd")
+import klft_amd as klft  # noqa: E402
 
 
 @pytest.fixture(scope="module", params=[8, 32])

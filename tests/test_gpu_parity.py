@@ -12,7 +12,7 @@ import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
-klft = pytest.importorskip("klft_amd")
+import klft_amd as klft  # noqa: E402
 
 SHAPES = [(2, (6, 4)), (3, (4, 6, 4)), (4, (4, 4, 6, 4))]
 
@@ -35,6 +35,45 @@ def test_random_algebra_init_matches_oracle(nd, nc, dims):
     U = seeded(nd, nc, dims, 0.5, 7)
     g = klft.GaugeField(nd, nc, dims, "f64").set_random_algebra(0.5, 7)
     np.testing.assert_allclose(g.download(), U, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f32_rows"])
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", SHAPES)
+def test_random_sun_init_matches_oracle(nd, nc, dims, storage):
+    """deviceGaugeField(L.., rng, delta): every link randSUN(delta)
+    (GaugeField.hpp:96-112); FP64 within 1e-14, FP32 storage within the
+    rounding of the stored precision."""
+    U = orc.random_sun_field(nd, nc, dims, 0.5, 2468)
+    g = klft.GaugeField(nd, nc, dims, storage).set_random_sun(0.5, 2468)
+    tol = 1e-14 if storage == "f64" else 1e-6
+    np.testing.assert_allclose(g.download(), U, rtol=0, atol=tol)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("nc", [1, 2, 3])
+@pytest.mark.parametrize("nd,dims", SHAPES)
+def test_random_complex_and_constant_init(nd, nc, dims, storage):
+    """deviceGaugeField(L.., rng) (GaugeField.hpp:118-141): every entry
+    complex(drand(-1,1), drand(-1,1)), bit-exact in FP64; and
+    deviceGaugeField(L.., complex_t) (:65-84): every entry the constant."""
+    U = orc.random_complex_field(nd, nc, dims, 97)
+    g = klft.GaugeField(nd, nc, dims, storage).set_random_complex(97)
+    if storage == "f64":
+        assert np.array_equal(g.download(), U)
+    else:
+        np.testing.assert_allclose(g.download(), U, rtol=0, atol=1e-7)
+    g.set_constant(0.25 - 0.5j)
+    assert np.array_equal(g.download(), np.full(U.shape, 0.25 - 0.5j))
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_general_matrix_inits_reject_two_row_storage(nc):
+    g = klft.GaugeField(4, nc, (4, 4, 4, 4), "f32_rows")
+    with pytest.raises(klft.KlftError):
+        g.set_random_complex(1)
+    with pytest.raises(klft.KlftError):
+        g.set_constant(1.0)
 
 
 @pytest.mark.parametrize("nd,dims", SHAPES + [(4, (8, 6, 4, 10))])
@@ -97,9 +136,11 @@ def test_wilson_mu_nu_parity(nd, nc, dims):
 
 @pytest.mark.parametrize("nc", [1, 2, 3])
 def test_wilson_temporal_parity(nc):
-    nd, dims = 4, (4, 6, 4, 8)
+    """Every L, T in 1..8 on 8 x 6 x 10 x 12 (loops up to and beyond the
+    extents: lines wrap periodically)."""
+    nd, dims = 4, (8, 6, 10, 12)
     U = seeded(nd, nc, dims, 0.4, 3)
-    pairs = [(2, 2), (3, 4), (4, 3), (4, 4)]
+    pairs = [(L, T) for L in range(1, 9) for T in range(1, 9)]
     ref = orc.wilson_temporal(U, nd, nc, dims, pairs)
     g = klft.GaugeField(nd, nc, dims, "f64").upload(U)
     got = klft.WilsonLoop_temporal(g, pairs)

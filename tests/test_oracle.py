@@ -416,3 +416,54 @@ def test_defect_links_at_zero_coupling_accept_every_proposal():
     np.testing.assert_allclose(Ud[on], Uf[on], rtol=0, atol=1e-14)
     # links off the defect at beta = 6 reject some proposals, so they differ
     assert not np.allclose(Ud[~on], Uf[~on])
+
+
+# ---------------------------------------------------------------------------
+# field starts: closed forms from the keyed stream's uniforms
+# ---------------------------------------------------------------------------
+def test_random_complex_start_entries_are_the_stream_uniforms():
+    """GaugeField.hpp:118-141: entry (c1, c2) = complex(-1 + 2 u_{2k},
+    -1 + 2 u_{2k+1}) with k = c1 * nc + c2, drawn from the link's stream
+    (tag 6, c0 = lexicographic site, mu)."""
+    nd, dims = 4, (2, 4, 2, 4)
+    for nc in (1, 2, 3):
+        U = orc.random_complex_field(nd, nc, dims, 31)
+        V = int(np.prod(dims))
+        for site in [0, 5, V - 1]:
+            for mu in range(nd):
+                u = orc.stream_doubles(31, 6, site, 0, 0, 0, mu, 2 * nc * nc)
+                want = (-1.0 + 2.0 * u[0::2]) + 1j * (-1.0 + 2.0 * u[1::2])
+                got = U.reshape(V, nd, nc, nc)[site, mu].reshape(-1)
+                assert np.array_equal(got, want)
+        assert np.all(np.abs(U.real) <= 1) and np.all(np.abs(U.imag) <= 1)
+
+
+def test_random_sun_start_follows_the_proposal_laws():
+    """GaugeField.hpp:96-112: every link is randSUN(delta) drawn from its
+    own stream (tag 5).  U(1): phase = delta pi (2 u0 - 1).  SU(2): Re U00 =
+    cos(2 pi delta u0), Im U00 = (2 u1 - 1) sin(2 pi delta u0).  SU(3): the
+    first row is the first six uniforms (times delta) normalised (the
+    rejection loop repeats only for a norm below 2^-53).  Every link
+    unitary, with det 1 for Nc > 1."""
+    nd, dims, delta, seed = 4, (2, 2, 4, 2), 0.4, 77
+    V = int(np.prod(dims))
+    for nc in (1, 2, 3):
+        U = orc.random_sun_field(nd, nc, dims, delta, seed).reshape(V, nd, nc, nc)
+        for site in [0, 3, V - 1]:
+            for mu in range(nd):
+                u = orc.stream_doubles(seed, 5, site, 0, 0, 0, mu, 6)
+                a = U[site, mu]
+                if nc == 1:
+                    assert abs(np.angle(a[0, 0]) - delta * np.pi * (2 * u[0] - 1)) < 1e-14
+                elif nc == 2:
+                    alpha = 2 * np.pi * delta * u[0]
+                    assert abs(a[0, 0].real - np.cos(alpha)) < 1e-14
+                    assert abs(a[0, 0].imag - (2 * u[1] - 1) * np.sin(alpha)) < 1e-14
+                else:
+                    v = delta * u
+                    row = (v[0::2] + 1j * v[1::2]) / np.linalg.norm(v)
+                    np.testing.assert_allclose(a[0], row, rtol=0, atol=1e-15)
+        M = U.reshape(-1, nc, nc)
+        assert np.abs(M @ np.conj(np.transpose(M, (0, 2, 1))) - np.eye(nc)).max() < 1e-14
+        if nc > 1:  # U(1) links are phases: unitary, not special
+            assert np.abs(np.linalg.det(M) - 1).max() < 1e-13
