@@ -91,6 +91,14 @@ int klft_field_reunitarize(klft_field_t f);
  * per_site: optional host array of V doubles receiving Re p(x) in
  * lexicographic order. */
 int klft_plaquette(klft_field_t f, int normalize, double *out, double *per_site);
+/* klft_plaquette without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The reduction
+ * workspace and the per-site staging come from one static device array per
+ * GPU (8.5 MiB, held until the process ends) instead of the field's workspace
+ * and a per-call hipMalloc; per_site requires V <= 2^20 sites (KLFT_E_ARG
+ * beyond).  Calls that overlap on one GPU share that array and race. */
+int klft_plaquette_noalloc(klft_field_t f, int normalize, double *out, double *per_site);
 /* stapleField (Gauge_Util.hpp:27-97): out must be a full-storage field of the
  * same shape and precision; out(x,mu) = staple(x,mu) (GaugeField.hpp:187-237). */
 int klft_staple_field(klft_field_t f, klft_field_t out);

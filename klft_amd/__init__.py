@@ -106,6 +106,7 @@ _SIGS = {
     "klft_field_copy": (C.c_int, [_p, _p]),
     "klft_field_reunitarize": (C.c_int, [_p]),
     "klft_plaquette": (C.c_int, [_p, C.c_int, _dp, _dp]),
+    "klft_plaquette_noalloc": (C.c_int, [_p, C.c_int, _dp, _dp]),
     "klft_staple_field": (C.c_int, [_p, _p]),
     "klft_wilson_loop_mu_nu": (C.c_int, [_p, C.c_int, C.c_int, _ip, C.c_int, C.c_int, _dp, _dp]),
     "klft_wilson_loop_temporal": (C.c_int, [_p, _ip, C.c_int, C.c_int, _dp]),

@@ -596,7 +596,15 @@ __global__ __launch_bounds__(256) void k_plaquette_walk(Links<R, NC, COMP> f, Ge
 
 namespace klft_internal {
 // raw (unnormalised) plaquette sum into f->result[slot]; optional per-site
-static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int variant) {
+// reduction workspace of one launch: per-block partials (capacity cap), the
+// grid ticket and the result slot
+struct RedBuf {
+  double *partials;
+  long long cap;
+  unsigned int *ticket;
+  double *out;
+};
+static int launch_plaquette(klft_field_t f, const RedBuf &rb, double *dev_per_site, int variant) {
   return dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
     if (variant == 4 || variant == 5) {
@@ -609,25 +617,25 @@ static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int 
       const int S = (f->g.L[0] + nseg - 1) / nseg;
       const int nsegs = (f->g.L[0] + S - 1) / S;
       const unsigned nb = (unsigned)(nsegs * ((Vs + 255) / 256));
-      if ((long long)nb > f->max_partials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
+      if ((long long)nb > rb.cap) return set_error(KLFT_E_ARG, "lattice too large for reduction");
       ProfScope prof("plaquette");
       if (variant == 4)
         hipLaunchKernelGGL((k_plaquette_walk<R, NC, COMP, ND, true>), dim3(nb), dim3(256), 0, stream(),
-                           links_of<R, NC, COMP>(f), f->g, S, Vs, f->partials, f->ticket, f->result + slot,
+                           links_of<R, NC, COMP>(f), f->g, S, Vs, rb.partials, rb.ticket, rb.out,
                            dev_per_site);
       else
         hipLaunchKernelGGL((k_plaquette_walk<R, NC, COMP, ND, false>), dim3(nb), dim3(256), 0, stream(),
-                           links_of<R, NC, COMP>(f), f->g, S, Vs, f->partials, f->ticket, f->result + slot,
+                           links_of<R, NC, COMP>(f), f->g, S, Vs, rb.partials, rb.ticket, rb.out,
                            dev_per_site);
       KLFT_HIP(hipGetLastError());
       return KLFT_OK;
     }
     const unsigned nb = blocks_for(chunk_threads<ND>(f->g), 256);
-    if ((long long)nb > f->max_partials) return set_error(KLFT_E_ARG, "lattice too large for reduction");
+    if ((long long)nb > rb.cap) return set_error(KLFT_E_ARG, "lattice too large for reduction");
     if (variant == 3) {
       ProfScope prof("plaquette");
       hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, 1, true>), dim3(nb), dim3(256), 0, stream(),
-                         links_of<R, NC, COMP>(f), f->g, f->partials, f->ticket, f->result + slot, dev_per_site);
+                         links_of<R, NC, COMP>(f), f->g, rb.partials, rb.ticket, rb.out, dev_per_site);
       KLFT_HIP(hipGetLastError());
       return KLFT_OK;
     }
@@ -635,19 +643,21 @@ static int launch_plaquette(klft_field_t f, int slot, double *dev_per_site, int 
       constexpr int OCC = decltype(occ)::value;
       ProfScope prof("plaquette");
       hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, OCC>), dim3(nb), dim3(256), 0, stream(),
-                         links_of<R, NC, COMP>(f), f->g, f->partials, f->ticket, f->result + slot,
+                         links_of<R, NC, COMP>(f), f->g, rb.partials, rb.ticket, rb.out,
                          dev_per_site);
       KLFT_HIP(hipGetLastError());
       return KLFT_OK;
     });
   });
 }
+// untuned defaults, the fastest at 32^4 and 48^4: threads walking dimension
+// 0, plane by plane for FP64 links, all loads in flight for FP32
+static int plaquette_default(const klft_field_s *f) { return f->storage == KLFT_STORAGE_F64 ? 5 : 4; }
 int plaquette_raw(klft_field_t f, int slot, double *dev_per_site) {
-  // untuned defaults, the fastest at 32^4 and 48^4: threads walking
-  // dimension 0, plane by plane for FP64 links, all loads in flight for FP32
-  const int v = pick_variant("plaquette", f, kOccVariants + 3, f->storage == KLFT_STORAGE_F64 ? 5 : 4,
-                             [&](int var) { return launch_plaquette(f, slot, nullptr, var); });
-  return launch_plaquette(f, slot, dev_per_site, v);
+  const RedBuf rb{f->partials, f->max_partials, f->ticket, f->result + slot};
+  const int v = pick_variant("plaquette", f, kOccVariants + 3, plaquette_default(f),
+                             [&](int var) { return launch_plaquette(f, rb, nullptr, var); });
+  return launch_plaquette(f, rb, dev_per_site, v);
 }
 int read_results(klft_field_t f, int first, int n, double *out) {
   KLFT_HIP(hipMemcpyAsync(f->host_result, f->result + first, n * sizeof(double), hipMemcpyDeviceToHost,
@@ -673,6 +683,84 @@ extern "C" int klft_plaquette(klft_field_t f, int normalize, double *out, double
   }
   if (dps) (void)hipFree(dps);
   if (rc) return rc;
+  if (normalize) {  // GaugePlaquette.hpp:125-132
+    double norm = 1.0;
+    for (int d = 0; d < f->nd; ++d) norm *= (double)f->dims[d];
+    norm *= (double)((f->nd * (f->nd - 1) / 2) * f->nc);
+    sum /= norm;
+  }
+  *out = sum;
+  return KLFT_OK;
+}
+
+// ============================================================================
+// plaquette without allocation, for callers that cannot allocate device
+// memory.  Its reduction partials, grid ticket, result and per-site staging
+// live in `scratch`, one static array per GPU (held until the process
+// ends), in place of the field's reduction workspace and the per-call
+// hipMalloc of klft_plaquette.  Calls that overlap on one GPU share the array
+// and race.
+// ============================================================================
+namespace {
+constexpr long long kNaPartials = 1 << 16;  // reduction blocks
+constexpr long long kNaSlots = 32;          // result (slot 0) and ticket (slot 16)
+constexpr long long kNaSites = 1 << 20;     // per-site staging (32^4)
+static __device__ __align__(256) double scratch[kNaPartials + kNaSlots + kNaSites];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+double *g_na_ptr[kNaMaxDevices] = {};
+
+// the device address of this GPU's scratch (the array's name is a host-side
+// stand-in and must never reach a kernel), resolved once per device with the
+// device current
+int scratch_of_current(double **out) {
+  int dev = -1;
+  KLFT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices) return ki::set_error(KLFT_E_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void *p = nullptr;
+    KLFT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_na_ptr[dev];
+  return KLFT_OK;
+}
+
+// zero the result slot and the grid ticket (a kernel, not a memset node, so a
+// captured call replays correctly)
+__global__ void k_na_clear(double *out, unsigned int *ticket) {
+  if (threadIdx.x == 0) {
+    *out = 0.0;
+    *ticket = 0u;
+  }
+}
+}  // namespace
+
+extern "C" int klft_plaquette_noalloc(klft_field_t f, int normalize, double *out, double *per_site) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (!out) return ki::set_error(KLFT_E_ARG, "null output");
+  if (per_site && f->g.V > kNaSites)
+    return ki::set_error(KLFT_E_ARG, "per-site output of %lld sites exceeds the no-allocation limit of %lld",
+                         f->g.V, kNaSites);
+  double *base = nullptr;
+  rc = scratch_of_current(&base);
+  if (rc) return rc;
+  double *res = base + kNaPartials;
+  unsigned int *ticket = reinterpret_cast<unsigned int *>(base + kNaPartials + 16);
+  double *stage = per_site ? base + kNaPartials + kNaSlots : nullptr;
+  hipLaunchKernelGGL(k_na_clear, dim3(1), dim3(64), 0, ki::stream(), res, ticket);
+  KLFT_HIP(hipGetLastError());
+  const ki::RedBuf rb{base, kNaPartials, ticket, res};
+  rc = ki::launch_plaquette(f, rb, stage, ki::plaquette_default(f));
+  if (rc) return rc;
+  KLFT_HIP(hipMemcpyAsync(f->host_result, res, sizeof(double), hipMemcpyDeviceToHost, ki::stream()));
+  if (per_site)
+    KLFT_HIP(hipMemcpyAsync(per_site, stage, sizeof(double) * f->g.V, hipMemcpyDeviceToHost, ki::stream()));
+  KLFT_HIP(hipStreamSynchronize(ki::stream()));
+  double sum = f->host_result[0];
   if (normalize) {  // GaugePlaquette.hpp:125-132
     double norm = 1.0;
     for (int d = 0; d < f->nd; ++d) norm *= (double)f->dims[d];
