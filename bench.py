@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--nhits", type=int, default=10)
     ap.add_argument("--beta", type=float, default=6.0)
     ap.add_argument("--delta", type=float, default=0.1)
-    ap.add_argument("--hmc-traj", type=int, default=3)
+    ap.add_argument("--hmc-traj", type=int, default=10)
+    ap.add_argument("--hmc-warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="Metropolis lines only")
     ap.add_argument("--copy-probe", action="store_true",
@@ -84,18 +85,29 @@ def metropolis_leg(klft, st, a, dims):
 
 
 def hmc_leg(klft, st, a, dims):
+    """HMC trajectories (tau = 1, 20 leapfrog steps = 21 fused MD steps) at
+    beta, after a.hmc_warmup untimed trajectories from the exp(i 0.5 H) start
+    have brought the action near its equilibrium value."""
+    import numpy as np
     g = klft.GaugeField(4, 3, dims, st).set_random_algebra(0.5, 77)
     h = klft.HMC(g, beta=a.beta, tau=1.0, nsteps=20, seed=99)
-    h.trajectory(0)
+    for i in range(a.hmc_warmup):
+        h.trajectory(i)
     klft.profile(True)
     res = []
-    t = sync_time(lambda: res.extend(h.trajectory(i + 1) for i in range(a.hmc_traj)))
+    t = sync_time(lambda: res.extend(h.trajectory(a.hmc_warmup + i) for i in range(a.hmc_traj)))
+    mms, mn = klft.profile_read("md_step")
     fms, fn = klft.profile_read("force")
-    ums, un = klft.profile_read("update_gauge")
     klft.profile(False)
-    return {"traj_per_s": a.hmc_traj / t, "ms_per_traj": 1e3 * t / a.hmc_traj,
-            "dH": [r[0] for r in res], "accepted": [r[1] for r in res],
-            "force_us": 1e3 * fms / max(fn, 1), "update_gauge_us": 1e3 * ums / max(un, 1)}
+    dH = np.array([r[0] for r in res])
+    out = {"traj_per_s": a.hmc_traj / t, "ms_per_traj": 1e3 * t / a.hmc_traj, "trajectories": a.hmc_traj,
+           "warmup_trajectories": a.hmc_warmup, "acceptance": float(np.mean([r[1] for r in res])),
+           "mean_dH": float(dH.mean()), "mean_abs_dH": float(np.abs(dH).mean()),
+           "plaquette": klft.GaugePlaquette(g),
+           "md_step_us": 1e3 * mms / max(mn, 1), "final_half_step_force_us": 1e3 * fms / max(fn, 1)}
+    h.close()
+    g.close()
+    return out
 
 
 def measure_leg(klft, g, st, dims):

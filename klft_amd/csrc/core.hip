@@ -139,6 +139,7 @@ extern "C" const char *klft_version(void) { return "klft_hip 1.0 gfx950"; }
 template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(256) void k_upload(Links<R, NC, COMP> f, Geom g,
                                                  const double *__restrict__ host_img) {
+  f.nd = ND;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;  // lexicographic
   if (s >= g.V) return;
   int x[ND];
@@ -159,6 +160,7 @@ __global__ __launch_bounds__(256) void k_upload(Links<R, NC, COMP> f, Geom g,
 template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(256) void k_download(Links<R, NC, COMP> f, Geom g,
                                                    double *__restrict__ host_img) {
+  f.nd = ND;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (s >= g.V) return;
   int x[ND];
@@ -184,6 +186,7 @@ __global__ __launch_bounds__(256) void k_download(Links<R, NC, COMP> f, Geom g,
 template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(256) void k_init(Links<R, NC, COMP> f, Geom g, int kind,
                                                double param, double param2, uint64_t seed) {
+  f.nd = ND;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;  // lexicographic
   if (s >= g.V) return;
   int x[ND];
@@ -277,6 +280,7 @@ __global__ __launch_bounds__(256) void k_init(Links<R, NC, COMP> f, Geom g, int 
 
 template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(256) void k_reunit(Links<R, NC, COMP> f, Geom g) {
+  f.nd = ND;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (s >= g.V) return;
 #pragma unroll
@@ -289,6 +293,8 @@ __global__ __launch_bounds__(256) void k_reunit(Links<R, NC, COMP> f, Geom g) {
 
 template <typename RO, int NC, bool CO, typename RI, bool CI, int ND>
 __global__ __launch_bounds__(256) void k_convert(Links<RO, NC, CO> o, Links<RI, NC, CI> in, Geom g) {
+  o.nd = ND;
+  in.nd = ND;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (s >= g.V) return;
 #pragma unroll
@@ -533,6 +539,7 @@ template <typename R, int NC, bool COMP, int ND, int OCC, bool FULL = false>
 __global__ __launch_bounds__(256, OCC) void k_plaquette(Links<R, NC, COMP> f, Geom g, double *partials,
                                                     unsigned int *ticket, double *out,
                                                     double *per_site) {
+  f.nd = ND;
   __shared__ double lds[8];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   int x[ND];
@@ -562,6 +569,7 @@ template <typename R, int NC, bool COMP, int ND, bool FULL>
 __global__ __launch_bounds__(256) void k_plaquette_walk(Links<R, NC, COMP> f, Geom g, int S, long long Vs,
                                                         double *partials, unsigned int *ticket, double *out,
                                                         double *per_site) {
+  f.nd = ND;
   __shared__ double lds[8];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   const unsigned int nreg = (unsigned int)((Vs + 255) / 256);
@@ -779,6 +787,8 @@ extern "C" int klft_plaquette_noalloc(klft_field_t f, int normalize, double *out
 template <typename R, int NC, bool COMP, int ND, int OCC, bool LOWREG = true>
 __global__ __launch_bounds__(256, OCC) void k_staple_field(Links<R, NC, COMP> f,
                                                                             Links<R, NC, false> o, Geom g) {
+  f.nd = ND;
+  o.nd = ND;
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   int x[ND], mu;
   long long s;
@@ -846,6 +856,7 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
                                                  int ncolblk, int nL, const int *__restrict__ Ls,
                                                  int Tmax, double *partials, double *per_site,
                                                  int ps_li, int ps_T) {
+  f.nd = ND;
   extern __shared__ __align__(16) unsigned char smem[];
   Mat<R, NC> *lds = reinterpret_cast<Mat<R, NC> *>(smem);
   const int plane = blockIdx.y;
@@ -917,6 +928,21 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
                  (tid >> 6)] = ws;
     }
   }
+}
+
+// fixed-order sum of n block partials into one output
+__global__ __launch_bounds__(256) void k_sum_blocks(const double *partials, long long n, double *out) {
+  __shared__ double red[8];
+  double v = 0.0;
+  for (long long i = threadIdx.x; i < n; i += 256) v += partials[i];
+  v = block_sum<256>(v, red);
+  if (threadIdx.x == 0) *out = v;
+}
+
+int klft_internal::enqueue_sum_blocks(const double *partials, long long n, double *out) {
+  hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(256), 0, ki::stream(), partials, n, out);
+  KLFT_HIP(hipGetLastError());
+  return KLFT_OK;
 }
 
 // fixed-order sum of nblk partials for each output

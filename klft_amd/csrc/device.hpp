@@ -98,7 +98,41 @@ __device__ __forceinline__ Mat<R, N> identity() {
   return z;
 }
 
-// a * b, k accumulated in order (SUN.hpp:28-44)
+// Products of Nc x Nc matrices, k accumulated in order (SUN.hpp:28-44).  Each
+// complex multiply-add is four FMAs: the first product of an entry is formed
+// as mul + fma per component and every further term is fused into the
+// accumulator, which rounds differently from the reference's separately
+// rounded product and sum only at the last bit.
+template <typename R>
+__device__ __forceinline__ R fmar(R a, R b, R c) {
+  return fma(a, b, c);
+}
+// t + a * b
+template <typename R>
+__device__ __forceinline__ cx<R> cfma(cx<R> a, cx<R> b, cx<R> t) {
+  return mk(fmar(-a.im, b.im, fmar(a.re, b.re, t.re)), fmar(a.im, b.re, fmar(a.re, b.im, t.im)));
+}
+// t + a * conj(b)
+template <typename R>
+__device__ __forceinline__ cx<R> cfma_c(cx<R> a, cx<R> b, cx<R> t) {
+  return mk(fmar(a.im, b.im, fmar(a.re, b.re, t.re)), fmar(-a.re, b.im, fmar(a.im, b.re, t.im)));
+}
+// t + conj(a) * b
+template <typename R>
+__device__ __forceinline__ cx<R> cfma_l(cx<R> a, cx<R> b, cx<R> t) {
+  return mk(fmar(a.im, b.im, fmar(a.re, b.re, t.re)), fmar(-a.im, b.re, fmar(a.re, b.im, t.im)));
+}
+// t + conj(a) * conj(b)
+template <typename R>
+__device__ __forceinline__ cx<R> cfma_cc(cx<R> a, cx<R> b, cx<R> t) {
+  return mk(fmar(-a.im, b.im, fmar(a.re, b.re, t.re)), fmar(-a.im, b.re, fmar(-a.re, b.im, t.im)));
+}
+// a * b alone (mul + fma per component)
+template <typename R>
+__device__ __forceinline__ cx<R> cmul1(cx<R> a, cx<R> b) {
+  return mk(fmar(a.re, b.re, -(a.im * b.im)), fmar(a.re, b.im, a.im * b.re));
+}
+
 template <typename R, int N>
 __device__ __forceinline__ Mat<R, N> mul(const Mat<R, N>& a, const Mat<R, N>& b) {
   Mat<R, N> c;
@@ -106,9 +140,9 @@ __device__ __forceinline__ Mat<R, N> mul(const Mat<R, N>& a, const Mat<R, N>& b)
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      cx<R> t = a.m[i][0] * b.m[0][j];
+      cx<R> t = cmul1(a.m[i][0], b.m[0][j]);
 #pragma unroll
-      for (int k = 1; k < N; ++k) t = t + a.m[i][k] * b.m[k][j];
+      for (int k = 1; k < N; ++k) t = cfma(a.m[i][k], b.m[k][j], t);
       c.m[i][j] = t;
     }
   return c;
@@ -121,9 +155,9 @@ __device__ __forceinline__ Mat<R, N> mul_ad(const Mat<R, N>& a, const Mat<R, N>&
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      cx<R> t = mulc(a.m[i][0], b.m[j][0]);
+      cx<R> t = cmul1(a.m[i][0], conj(b.m[j][0]));
 #pragma unroll
-      for (int k = 1; k < N; ++k) t = t + mulc(a.m[i][k], b.m[j][k]);
+      for (int k = 1; k < N; ++k) t = cfma_c(a.m[i][k], b.m[j][k], t);
       c.m[i][j] = t;
     }
   return c;
@@ -136,9 +170,9 @@ __device__ __forceinline__ Mat<R, N> mul_da(const Mat<R, N>& a, const Mat<R, N>&
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      cx<R> t = cmul_(a.m[0][i], b.m[0][j]);
+      cx<R> t = cmul1(conj(a.m[0][i]), b.m[0][j]);
 #pragma unroll
-      for (int k = 1; k < N; ++k) t = t + cmul_(a.m[k][i], b.m[k][j]);
+      for (int k = 1; k < N; ++k) t = cfma_l(a.m[k][i], b.m[k][j], t);
       c.m[i][j] = t;
     }
   return c;
@@ -151,9 +185,9 @@ __device__ __forceinline__ Mat<R, N> mul_dd(const Mat<R, N>& a, const Mat<R, N>&
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      cx<R> t = mulcc(a.m[0][i], b.m[j][0]);
+      cx<R> t = cmul1(conj(a.m[0][i]), conj(b.m[j][0]));
 #pragma unroll
-      for (int k = 1; k < N; ++k) t = t + mulcc(a.m[k][i], b.m[j][k]);
+      for (int k = 1; k < N; ++k) t = cfma_cc(a.m[k][i], b.m[j][k], t);
       c.m[i][j] = t;
     }
   return c;
@@ -165,14 +199,39 @@ __device__ __forceinline__ void add_to(Mat<R, N>& a, const Mat<R, N>& b) {
 #pragma unroll
     for (int j = 0; j < N; ++j) a.m[i][j] = a.m[i][j] + b.m[i][j];
 }
-// Re Tr(a * b)
+// t += a * b and t += a * b^dagger, each product term fused into t
+template <typename R, int N>
+__device__ __forceinline__ void acc_mul(Mat<R, N>& t, const Mat<R, N>& a, const Mat<R, N>& b) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> v = t.m[i][j];
+#pragma unroll
+      for (int k = 0; k < N; ++k) v = cfma(a.m[i][k], b.m[k][j], v);
+      t.m[i][j] = v;
+    }
+}
+template <typename R, int N>
+__device__ __forceinline__ void acc_mul_ad(Mat<R, N>& t, const Mat<R, N>& a, const Mat<R, N>& b) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cx<R> v = t.m[i][j];
+#pragma unroll
+      for (int k = 0; k < N; ++k) v = cfma_c(a.m[i][k], b.m[j][k], v);
+      t.m[i][j] = v;
+    }
+}
+// Re Tr(a * b), two FMAs per term
 template <typename R, int N>
 __device__ __forceinline__ R retr_mul(const Mat<R, N>& a, const Mat<R, N>& b) {
   R t = R(0);
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int k = 0; k < N; ++k) t += a.m[i][k].re * b.m[k][i].re - a.m[i][k].im * b.m[k][i].im;
+    for (int k = 0; k < N; ++k) t = fmar(-a.m[i][k].im, b.m[k][i].im, fmar(a.m[i][k].re, b.m[k][i].re, t));
   return t;
 }
 // Re Tr(a * b^dagger) = sum_ij Re(a_ij conj(b_ij)), summed c1 outer, c2 inner
@@ -182,7 +241,7 @@ __device__ __forceinline__ R retr_mul_ad(const Mat<R, N>& a, const Mat<R, N>& b)
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < N; ++j) t += a.m[i][j].re * b.m[i][j].re + a.m[i][j].im * b.m[i][j].im;
+    for (int j = 0; j < N; ++j) t = fmar(a.m[i][j].im, b.m[i][j].im, fmar(a.m[i][j].re, b.m[i][j].re, t));
   return t;
 }
 template <typename R, int N>
@@ -484,7 +543,8 @@ struct Links {
   static constexpr int NCOMP = C ? (NC - 1) * NC : NC * NC;
   cx<R>* __restrict__ p;
   long long V;
-  int nd;
+  int nd;  // kernels templated on the direction count set nd = ND on entry,
+           // which makes the index arithmetic below compile-time
 
   __device__ __forceinline__ long long base(long long s, int mu) const {
     return (((s >> 6) * nd + mu) * NCOMP) * 64 + (s & 63);
@@ -530,7 +590,7 @@ __device__ __forceinline__ Mat<R, NC> staple(const Links<R, NC, COMP>& f, const 
     const Mat<R, NC> a = f.load(s_xpmu, nu);
     const Mat<R, NC> b = f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu);
     const Mat<R, NC> c = f.load(s_x, nu);
-    add_to(t, mul_ad(mul_ad(a, b), c));
+    acc_mul_ad(t, mul_ad(a, b), c);
   }
 #pragma unroll
   for (int nu = 0; nu < ND; ++nu) {
@@ -539,7 +599,7 @@ __device__ __forceinline__ Mat<R, NC> staple(const Links<R, NC, COMP>& f, const 
     const Mat<R, NC> a = f.load(idx_shift<ND>(g, x, mu, 1, nu, -1), nu);
     const Mat<R, NC> b = f.load(s_xmnu, mu);
     const Mat<R, NC> c = f.load(s_xmnu, nu);
-    add_to(t, mul(mul_dd(a, b), c));
+    acc_mul(t, mul_dd(a, b), c);
   }
   return t;
 }
@@ -556,7 +616,7 @@ __device__ __forceinline__ Mat<R, NC> staple(const Links<R, NC, COMP>& f, const 
     const Mat<R, NC> a = f.load(s_xpmu, nu);
     const Mat<R, NC> b = f.load(n.up(nu), mu);
     const Mat<R, NC> c = f.load(n.s, nu);
-    add_to(t, mul_ad(mul_ad(a, b), c));
+    acc_mul_ad(t, mul_ad(a, b), c);
   }
 #pragma unroll
   for (int nu = 0; nu < ND; ++nu) {
@@ -565,7 +625,7 @@ __device__ __forceinline__ Mat<R, NC> staple(const Links<R, NC, COMP>& f, const 
     const Mat<R, NC> a = f.load(n.updn(mu, nu), nu);
     const Mat<R, NC> b = f.load(s_xmnu, mu);
     const Mat<R, NC> c = f.load(s_xmnu, nu);
-    add_to(t, mul(mul_dd(a, b), c));
+    acc_mul(t, mul_dd(a, b), c);
   }
   return t;
 }
@@ -583,14 +643,14 @@ __device__ __forceinline__ Mat<R, NC> staple_lowreg(const Links<R, NC, COMP>& f,
   for (int nu = 0; nu < ND; ++nu) {
     if (nu == mu) continue;
     const Mat<R, NC> ab = mul_ad(f.load(s_xpmu, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
-    add_to(t, mul_ad(ab, f.load(s_x, nu)));
+    acc_mul_ad(t, ab, f.load(s_x, nu));
   }
 #pragma unroll 1
   for (int nu = 0; nu < ND; ++nu) {
     if (nu == mu) continue;
     const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
     const Mat<R, NC> ab = mul_dd(f.load(idx_shift<ND>(g, x, mu, 1, nu, -1), nu), f.load(s_xmnu, mu));
-    add_to(t, mul(ab, f.load(s_xmnu, nu)));
+    acc_mul(t, ab, f.load(s_xmnu, nu));
   }
   return t;
 }
@@ -602,14 +662,14 @@ __device__ __forceinline__ Mat<R, NC> staple_lowreg(const Links<R, NC, COMP>& f,
   for (int nu = 0; nu < ND; ++nu) {
     if (nu == mu) continue;
     const Mat<R, NC> ab = mul_ad(f.load(s_xpmu, nu), f.load(n.up(nu), mu));
-    add_to(t, mul_ad(ab, f.load(n.s, nu)));
+    acc_mul_ad(t, ab, f.load(n.s, nu));
   }
 #pragma unroll 1
   for (int nu = 0; nu < ND; ++nu) {
     if (nu == mu) continue;
     const long long s_xmnu = n.dn(nu);
     const Mat<R, NC> ab = mul_dd(f.load(n.updn(mu, nu), nu), f.load(s_xmnu, mu));
-    add_to(t, mul(ab, f.load(s_xmnu, nu)));
+    acc_mul(t, ab, f.load(s_xmnu, nu));
   }
   return t;
 }
@@ -952,8 +1012,7 @@ __device__ __forceinline__ Mat<R, 3> exp_iq(const R* v, Mat<R, 3>*) {
   q.m[2][1] = mk(v[5], v[6]);
   const Mat<R, 3> q2 = mul(q, q);
   const R c1 = R(0.5) * retrace(q2);
-  const Mat<R, 3> q3 = mul(q2, q);
-  const R c0 = retrace(q3) / R(3);
+  const R c0 = retr_mul(q2, q) / R(3);  // Tr Q^3 / 3 = det Q
   // a, b, c: coefficients of Q^n / n! ; f accumulates i^n Q^n / n!
   R a = 1, b = 0, c = 0;
   cx<R> f0 = mk<R>(1, 0), f1 = mk<R>(0, 0), f2 = mk<R>(0, 0);
@@ -1017,6 +1076,15 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   }
   __syncthreads();
   return t;
+}
+
+// Block partial for kernels that write large outputs: a grid_finish release
+// there would write back the XCD's dirty L2 lines once per block, so the
+// partials are summed by a second launch (k_sum_blocks) in block order.
+template <int BLOCK>
+__device__ __forceinline__ void block_partial(double v, double* lds, double* partials) {
+  const double bt = block_sum<BLOCK>(v, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bt;
 }
 
 // Single-pass grid reduction: every block writes its partial, the last block

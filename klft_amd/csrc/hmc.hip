@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "internal.hpp"
@@ -25,15 +26,17 @@ struct klft_hmc_s {
   klft_field_t f = nullptr;
   klft_hmc_params p{};
   void *P = nullptr;       // momenta (R)
-  void *Ubak = nullptr;    // saved links for a rejected trajectory
+  // the leapfrog's two link buffers: a trajectory reads the field's links
+  // and alternates between these, then hands the final buffer to the field
+  // and keeps the starting one, which a rejected trajectory hands back
+  void *Ubuf[2] = {nullptr, nullptr};
   double *Fdbg = nullptr;  // force in host (AoS) order, for klft_hmc_force
   double *stage = nullptr; // host-order FP64 momenta for upload / download
 };
 
 template <typename R, int NC, int ND>
 __global__ __launch_bounds__(256) void k_refresh(R *P, Geom g, uint64_t seed, uint32_t traj,
-                                                  double *partials, unsigned int *ticket,
-                                                  double *out) {
+                                                  double *partials) {
   __shared__ double lds[8];
   constexpr int NG = Ngen<NC>::value;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;  // device order
@@ -63,8 +66,7 @@ __global__ __launch_bounds__(256) void k_refresh(R *P, Geom g, uint64_t seed, ui
       k += 0.5 * n2;
     }
   }
-  const double bt = block_sum<256>(k, lds);
-  grid_finish<256>(bt, partials, ticket, out, lds);
+  block_partial<256>(k, lds, partials);
 }
 
 // P -= dtau * F with F recomputed from the current links.  Fout (optional):
@@ -73,8 +75,8 @@ __global__ __launch_bounds__(256) void k_refresh(R *P, Geom g, uint64_t seed, ui
 template <typename R, int NC, bool COMP, int ND, int OCC, bool LOWREG = true>
 __global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, Geom g,
                                                                      R coef, R dtau, double *Fout,
-                                                                     double *partials,
-                                                                     unsigned int *ticket, double *kin) {
+                                                                     double *partials) {
+  f.nd = ND;
   __shared__ double lds[8];
   constexpr int NG = Ngen<NC>::value;
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -107,10 +109,7 @@ __global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, 
       k = 0.5 * n2;
     }
   }
-  if (kin) {
-    const double bt = block_sum<256>(k, lds);
-    grid_finish<256>(bt, partials, ticket, kin, lds);
-  }
+  if (partials) block_partial<256>(k, lds, partials);
 }
 
 // The same update with one thread per site for all its directions, one
@@ -121,8 +120,8 @@ __global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, 
 // memory, where the one-thread-per-link kernel spreads them over four waves.
 template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(256) void k_force_site(Links<R, NC, COMP> f, R *P, Geom g, R coef, R dtau,
-                                                    double *Fout, double *partials, unsigned int *ticket,
-                                                    double *kin) {
+                                                    double *Fout, double *partials) {
+  f.nd = ND;
   __shared__ double lds[8];
   constexpr int NG = Ngen<NC>::value;
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -153,16 +152,53 @@ __global__ __launch_bounds__(256) void k_force_site(Links<R, NC, COMP> f, R *P, 
       }
     }
   }
-  if (kin) {
-    const double bt = block_sum<256>(k, lds);
-    grid_finish<256>(bt, partials, ticket, kin, lds);
+  if (partials) block_partial<256>(k, lds, partials);
+}
+
+// One leapfrog step with the momentum and link updates fused and the links
+// double-buffered: F from the links in src, P <- P - dtp F, and
+// dst(x,mu) = exp(i dtu P(x,mu)) src(x,mu) (FP32 links reunitarised).  The
+// staples read src only, so no link is written while another thread may
+// still read it; the arithmetic is that of k_force followed by
+// k_update_gauge, so the result is the same.  One thread per link as in
+// k_force: a block holds one 64-site chunk, wave w taking direction w.
+template <typename R, int NC, bool COMP, int ND, bool LOWREG, int OCC = 1>
+__global__ __launch_bounds__(256, OCC) void k_md_step(Links<R, NC, COMP> src, Links<R, NC, COMP> dst, R *P, Geom g,
+                                                  R coef, R dtp, R dtu) {
+  src.nd = ND;
+  dst.nd = ND;
+  constexpr int NG = Ngen<NC>::value;
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  int x[ND], mu;
+  long long s;
+  if (!link_site<ND>(g, lb, x, &s, &mu)) return;
+  const Nbr<ND> nb = make_nbr<ND>(g, x, s);
+  Mat<R, NC> S;
+  if constexpr (LOWREG)
+    S = staple_lowreg<R, NC, COMP, ND>(src, nb, mu);
+  else
+    S = staple<R, NC, COMP, ND>(src, nb, mu);
+  const Mat<R, NC> U = src.load(s, mu);
+  R v[NG];
+  project(mul(U, S), coef, v);
+#pragma unroll
+  for (int a = 0; a < NG; ++a) {
+    R *q = P + ((long long)mu * NG + a) * g.V + s;
+    const R pn = *q - dtp * v[a];
+    *q = pn;
+    v[a] = dtu * pn;
   }
+  const Mat<R, NC> e = exp_iq(v, (Mat<R, NC> *)nullptr);
+  Mat<R, NC> u = mul(e, U);
+  if constexpr (sizeof(R) == 4) reunitarize(u);
+  dst.store(s, mu, u);
 }
 
 // U <- exp(i dtau P) U; FP32 links are reunitarised after the update
 template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(256) void k_update_gauge(Links<R, NC, COMP> f, const R *__restrict__ P,
                                                        Geom g, R dtau) {
+  f.nd = ND;
   constexpr int NG = Ngen<NC>::value;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (s >= g.V) return;
@@ -236,9 +272,9 @@ int enqueue_refresh(klft_hmc_t h, uint64_t traj, int slot) {
     const unsigned nb = ki::blocks_for(f->g.V, 256);
     if ((long long)nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
     hipLaunchKernelGGL((k_refresh<R, NC, ND>), dim3(nb), dim3(256), 0, ki::stream(), (R *)h->P, f->g,
-                       h->p.seed, (uint32_t)traj, f->partials, f->ticket, f->result + slot);
+                       h->p.seed, (uint32_t)traj, f->partials);
     KLFT_HIP(hipGetLastError());
-    return KLFT_OK;
+    return ki::enqueue_sum_blocks(f->partials, nb, f->result + slot);
   });
 }
 
@@ -249,38 +285,39 @@ constexpr int kForceVariants = 5;
 
 int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int variant) {
   klft_field_t f = h->f;
+  // with a kinetic slot, every block writes a partial; k_sum_blocks adds them
+  double *parts = kin_slot >= 0 ? f->partials : nullptr;
   return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
-    if (variant == 3) {
-      const unsigned nbs = ki::blocks_for(chunk_threads<ND>(f->g), 256);
-      if ((long long)nbs > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+    const auto links = ki::links_of<R, NC, COMP>(f);
+    const R coef = (R)(h->p.beta / (double)NC);
+    long long nb = 0;
+    {
       ki::ProfScope prof("force");
-      hipLaunchKernelGGL((k_force_site<R, NC, COMP, ND>), dim3(nbs), dim3(256), 0, ki::stream(),
-                         ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC), (R)dtau,
-                         Fout, f->partials, f->ticket, kin_slot >= 0 ? f->result + kin_slot : (double *)nullptr);
+      if (variant == 3) {
+        nb = ki::blocks_for(chunk_threads<ND>(f->g), 256);
+        if (nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+        hipLaunchKernelGGL((k_force_site<R, NC, COMP, ND>), dim3((unsigned)nb), dim3(256), 0, ki::stream(), links,
+                           (R *)h->P, f->g, coef, (R)dtau, Fout, parts);
+      } else {
+        nb = link_blocks<ND>(f->g);
+        if (nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
+        const int rc = ki::with_occ(variant == 4 ? 0 : variant, [&](auto occ) -> int {
+          constexpr int OCC = decltype(occ)::value;
+          if (variant == 4)
+            hipLaunchKernelGGL((k_force<R, NC, COMP, ND, 1, false>), dim3((unsigned)nb), dim3(64 * ND), 0,
+                               ki::stream(), links, (R *)h->P, f->g, coef, (R)dtau, Fout, parts);
+          else
+            hipLaunchKernelGGL((k_force<R, NC, COMP, ND, OCC>), dim3((unsigned)nb), dim3(64 * ND), 0,
+                               ki::stream(), links, (R *)h->P, f->g, coef, (R)dtau, Fout, parts);
+          return KLFT_OK;
+        });
+        if (rc) return rc;
+      }
       KLFT_HIP(hipGetLastError());
-      return KLFT_OK;
     }
-    const unsigned nb = (unsigned)link_blocks<ND>(f->g);
-    if ((long long)nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
-    if (variant == 4) {
-      ki::ProfScope prof("force");
-      hipLaunchKernelGGL((k_force<R, NC, COMP, ND, 1, false>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
-                         ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC), (R)dtau,
-                         Fout, f->partials, f->ticket, kin_slot >= 0 ? f->result + kin_slot : (double *)nullptr);
-      KLFT_HIP(hipGetLastError());
-      return KLFT_OK;
-    }
-    return ki::with_occ(variant, [&](auto occ) -> int {
-      constexpr int OCC = decltype(occ)::value;
-      ki::ProfScope prof("force");
-      hipLaunchKernelGGL((k_force<R, NC, COMP, ND, OCC>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
-                         ki::links_of<R, NC, COMP>(f), (R *)h->P, f->g, (R)(h->p.beta / (double)NC),
-                         (R)dtau, Fout, f->partials, f->ticket,
-                         kin_slot >= 0 ? f->result + kin_slot : (double *)nullptr);
-      KLFT_HIP(hipGetLastError());
-      return KLFT_OK;
-    });
+    if (parts) return ki::enqueue_sum_blocks(parts, nb, f->result + kin_slot);
+    return KLFT_OK;
   });
 }
 
@@ -321,17 +358,64 @@ int enqueue_kinetic(klft_hmc_t h, int slot) {
   });
 }
 
+// variants: 0 one-term-at-a-time staple, 1 full staple (all loads in flight);
+// 2, 3 the same with >= 3 waves per SIMD requested
+constexpr int kMdVariants = 4;
+
+int launch_md_step(klft_hmc_t h, void *src, void *dst, double dtp, double dtu, int variant) {
+  klft_field_t f = h->f;
+  return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
+    KLFT_UNPACK(rt, nct, ct, ndt);
+    auto ls = ki::links_of<R, NC, COMP>(f), ld = ls;
+    ls.p = reinterpret_cast<cx<R> *>(src);
+    ld.p = reinterpret_cast<cx<R> *>(dst);
+    const unsigned nb = (unsigned)link_blocks<ND>(f->g);
+    const R coef = (R)(h->p.beta / (double)NC);
+    ki::ProfScope prof("md_step");
+    if (variant == 1)
+      hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, false>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,
+                         (R *)h->P, f->g, coef, (R)dtp, (R)dtu);
+    else if (variant == 2)
+      hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, true, 3>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,
+                         (R *)h->P, f->g, coef, (R)dtp, (R)dtu);
+    else if (variant == 3)
+      hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, false, 3>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,
+                         (R *)h->P, f->g, coef, (R)dtp, (R)dtu);
+    else
+      hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, true>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,
+                         (R *)h->P, f->g, coef, (R)dtp, (R)dtu);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
+  });
+}
+
 // Leapfrog::integrate (Integrator.hpp:26-53): dtau = tau/n; half momentum
 // step, gauge step, n x (momentum, gauge), half momentum step, restoreGauge.
-int enqueue_leapfrog(klft_hmc_t h, int kin_slot) {
+// Each momentum step and the gauge step after it are one fused launch that
+// writes the new links to the other buffer: field -> Ubuf[0] -> Ubuf[1] ->
+// Ubuf[0] ...  The final buffer is then swapped into the field, and the
+// starting links stay in the handle (slot `*start_slot`).
+int enqueue_leapfrog(klft_hmc_t h, int kin_slot, int *start_slot) {
   const double dtau = h->p.tau / (double)h->p.nsteps;
-  int rc = enqueue_force(h, dtau / 2.0, nullptr, -1);
-  if (!rc) rc = enqueue_gauge(h, dtau);
-  for (int i = 0; i < h->p.nsteps && !rc; ++i) {
-    rc = enqueue_force(h, dtau, nullptr, -1);
-    if (!rc) rc = enqueue_gauge(h, dtau);
+  klft_field_t f = h->f;
+  // untuned default: the staple form of the force kernel's default
+  const int v = ki::pick_variant("md_step", f, kMdVariants, f->storage == KLFT_STORAGE_F64 ? 1 : 0, [&](int var) {
+    return launch_md_step(h, f->links, h->Ubuf[0], 0.0, 0.0, var);  // dtp = 0: P unchanged
+  });
+  void *cur = f->links;
+  int nxt = 0, rc = KLFT_OK;
+  for (int i = 0; i <= h->p.nsteps && !rc; ++i) {
+    rc = launch_md_step(h, cur, h->Ubuf[nxt], i == 0 ? dtau / 2.0 : dtau, dtau, v);
+    cur = h->Ubuf[nxt];
+    nxt ^= 1;
   }
-  if (!rc) rc = enqueue_force(h, dtau / 2.0, nullptr, kin_slot);
+  if (rc) return rc;
+  // the field takes the final links; the buffer it held takes their slot
+  const int fin = nxt ^ 1;
+  h->Ubuf[fin] = f->links;
+  f->links = cur;
+  if (start_slot) *start_slot = fin;
+  rc = enqueue_force(h, dtau / 2.0, nullptr, kin_slot);
   if (!rc) rc = ki::enqueue_reunit(h->f);
   return rc;
 }
@@ -374,11 +458,12 @@ extern "C" int klft_hmc_create(klft_field_t f, const klft_hmc_params *p, klft_hm
   const size_t rbytes = f->storage == KLFT_STORAGE_F64 ? 8 : 4;
   const size_t pbytes = (size_t)f->g.V * f->nd * ngen(f->nc) * rbytes;
   hipError_t e = hipMalloc(&h->P, pbytes);
-  if (e == hipSuccess) e = hipMalloc(&h->Ubak, f->link_bytes);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipMalloc(&h->Ubuf[i], f->link_bytes);
   if (e == hipSuccess) e = hipMemset(h->P, 0, pbytes);
   if (e != hipSuccess) {
     if (h->P) (void)hipFree(h->P);
-    if (h->Ubak) (void)hipFree(h->Ubak);
+    for (void *b : h->Ubuf)
+      if (b) (void)hipFree(b);
     delete h;
     return ki::set_error(KLFT_E_NOMEM, "HMC workspace allocation failed");
   }
@@ -390,7 +475,8 @@ extern "C" int klft_hmc_destroy(klft_hmc_t h) {
   if (!h) return KLFT_OK;
   (void)ki::sync_stream();
   if (h->P) (void)hipFree(h->P);
-  if (h->Ubak) (void)hipFree(h->Ubak);
+  for (void *b : h->Ubuf)
+    if (b) (void)hipFree(b);
   if (h->Fdbg) (void)hipFree(h->Fdbg);
   if (h->stage) (void)hipFree(h->stage);
   h->P = nullptr;
@@ -469,7 +555,7 @@ extern "C" int klft_gauge_action(klft_field_t f, double beta, double *out) {
 
 extern "C" int klft_hmc_integrate(klft_hmc_t h) {
   int rc = check_hmc(h);
-  if (!rc) rc = enqueue_leapfrog(h, -1);
+  if (!rc) rc = enqueue_leapfrog(h, -1, nullptr);
   if (rc) return rc;
   return ki::sync_stream();
 }
@@ -482,9 +568,9 @@ extern "C" int klft_hmc_trajectory(klft_hmc_t h, uint64_t traj, double *dH, int 
   klft_field_t f = h->f;
   rc = enqueue_refresh(h, traj, 4);  // K_old
   if (rc) return rc;
-  KLFT_HIP(hipMemcpyAsync(h->Ubak, f->links, f->link_bytes, hipMemcpyDeviceToDevice, ki::stream()));
   rc = ki::plaquette_raw(f, 5, nullptr);  // G_old (raw)
-  if (!rc) rc = enqueue_leapfrog(h, 6);   // K_new fused into the last half step
+  int start = -1;                         // handle slot of the starting links
+  if (!rc) rc = enqueue_leapfrog(h, 6, &start);  // K_new fused into the last half step
   if (!rc) rc = ki::plaquette_raw(f, 7, nullptr);
   double r[4];
   if (!rc) rc = ki::read_results(f, 4, 4, r);
@@ -499,11 +585,7 @@ extern "C" int klft_hmc_trajectory(klft_hmc_t h, uint64_t traj, double *dH, int 
     klft_rng::Stream st(h->p.seed, klft_rng::TAG_ACCEPT, 0, (uint32_t)traj, 0, 0, 0);
     if (st.u01() > std::exp(-d)) acc = 0;
   }
-  if (!acc) {
-    KLFT_HIP(hipMemcpyAsync(f->links, h->Ubak, f->link_bytes, hipMemcpyDeviceToDevice, ki::stream()));
-    rc = ki::sync_stream();
-    if (rc) return rc;
-  }
+  if (!acc) std::swap(f->links, h->Ubuf[start]);  // the starting links, untouched
   if (dH) *dH = d;
   if (accepted) *accepted = acc;
   return KLFT_OK;

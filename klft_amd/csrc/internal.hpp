@@ -56,6 +56,9 @@ kd::Geom make_geom(int nd, const int *dims);
 int plaquette_raw(klft_field_t f, int slot, double *dev_per_site);  // sum into f->result[slot]
 int read_results(klft_field_t f, int first, int n, double *out);    // synchronises
 int enqueue_reunit(klft_field_t f);
+// out = sum of partials[0 .. n) in index order (one block, after the launch
+// that wrote them)
+int enqueue_sum_blocks(const double *partials, long long n, double *out);
 int enqueue_sweep(klft_field_t f, const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
                   unsigned long long *counter, int *visit);
 int check_field(const klft_field_s *f);

@@ -76,7 +76,7 @@ __device__ __forceinline__ Mat<R, NC> staple_c(const Links<R, NC, COMP> &f, cons
       const Mat<R, NC> a = load_at<R, NC, COMP, ND, true>(f, g, xp, nu, d, c);
       const Mat<R, NC> b = load_at<R, NC, COMP, ND, true>(f, g, y, mu, d, c);
       const Mat<R, NC> e = load_at<R, NC, COMP, ND, true>(f, g, x, nu, d, c);
-      add_to(t, mul_ad(mul_ad(a, b), e));
+      acc_mul_ad(t, mul_ad(a, b), e);
     }
 #pragma unroll
     for (int nu = 0; nu < ND; ++nu) {
@@ -87,7 +87,7 @@ __device__ __forceinline__ Mat<R, NC> staple_c(const Links<R, NC, COMP> &f, cons
       const Mat<R, NC> a = load_at<R, NC, COMP, ND, true>(f, g, ya, nu, d, c);
       const Mat<R, NC> b = load_at<R, NC, COMP, ND, true>(f, g, yb, mu, d, c);
       const Mat<R, NC> e = load_at<R, NC, COMP, ND, true>(f, g, yb, nu, d, c);
-      add_to(t, mul(mul_dd(a, b), e));
+      acc_mul(t, mul_dd(a, b), e);
     }
     return t;
   }
@@ -206,14 +206,14 @@ __device__ __forceinline__ void metro_dir_own(const Links<R, NC, COMP> &f, const
   for (int nu = 0; nu < ND; ++nu) {
     if (nu == MU) continue;
     const Mat<R, NC> ab = mul_ad(f.load(s_xpmu, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), MU));
-    add_to(S, mul_ad(ab, Uown[nu]));
+    acc_mul_ad(S, ab, Uown[nu]);
   }
 #pragma unroll 1
   for (int nu = 0; nu < ND; ++nu) {
     if (nu == MU) continue;
     const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
     const Mat<R, NC> ab = mul_dd(f.load(idx_shift<ND>(g, x, MU, 1, nu, -1), nu), f.load(s_xmnu, MU));
-    add_to(S, mul(ab, f.load(s_xmnu, nu)));
+    acc_mul(S, ab, f.load(s_xmnu, nu));
   }
   LinkChain<R, NC> lc;
   lc.start(S, Uown[MU]);
@@ -304,9 +304,9 @@ __device__ __forceinline__ void term_fold(int m, int t, int defer, const TermLoa
   const int j = upper ? t : t - (ND - 1);
   const int nu = j < m ? j : j + 1;
   if (upper) {
-    if (nu != defer) add_to(acc, mul_ad(mul_ad(L.a, L.b), L.c));
+    if (nu != defer) acc_mul_ad(acc, mul_ad(L.a, L.b), L.c);
   } else {
-    add_to(acc, mul(mul_dd(L.a, L.b), L.c));
+    acc_mul(acc, mul_dd(L.a, L.b), L.c);
   }
 }
 
@@ -363,9 +363,9 @@ __device__ __forceinline__ void term_fold_lds(int m, int t, int defer, const cx<
   const int j = upper ? t : t - (ND - 1);
   const int nu = j < m ? j : j + 1;
   if (upper) {
-    if (nu != defer) add_to(acc, mul_ad(mul_ad(lds_mat<NC>(lds, 0), lds_mat<NC>(lds, 1)), lds_mat<NC>(lds, 2)));
+    if (nu != defer) acc_mul_ad(acc, mul_ad(lds_mat<NC>(lds, 0), lds_mat<NC>(lds, 1)), lds_mat<NC>(lds, 2));
   } else {
-    add_to(acc, mul(mul_dd(lds_mat<NC>(lds, 0), lds_mat<NC>(lds, 1)), lds_mat<NC>(lds, 2)));
+    acc_mul(acc, mul_dd(lds_mat<NC>(lds, 0), lds_mat<NC>(lds, 1)), lds_mat<NC>(lds, 2));
   }
 }
 
@@ -423,7 +423,7 @@ __device__ __forceinline__ unsigned long long metro_site_pipe(const Links<R, NC,
       // the upper term through U_mu(x), now final
       const Mat<R, NC> ab = mul_ad(f.load(idx_shift<ND>(g, x, m, 1, -1, 0), mu),
                                    f.load(idx_shift<ND>(g, x, mu, 1, -1, 0), m));
-      add_to(acc, mul_ad(ab, lc.U));
+      acc_mul_ad(acc, ab, lc.U);
       S = acc;
     }
   }

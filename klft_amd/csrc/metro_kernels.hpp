@@ -24,6 +24,7 @@ template <typename R, int NC, bool COMP, int ND, bool LOWREG, int HB, int OCC>
 __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_pass(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  f.nd = ND;
   __shared__ unsigned long long red[kMetroBlock / 64];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   const long long t = (long long)lb * kMetroBlock + threadIdx.x;
@@ -47,6 +48,7 @@ template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  f.nd = ND;
   __shared__ unsigned long long red[kMetroBlock / 64];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   const long long t = (long long)lb * kMetroBlock + threadIdx.x;
@@ -69,6 +71,7 @@ template <typename R, int NC, bool COMP, int ND>
 __global__ __launch_bounds__(kMetroBlock) void k_metropolis_own(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  f.nd = ND;
   __shared__ unsigned long long red[kMetroBlock / 64];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   const long long t = (long long)lb * kMetroBlock + threadIdx.x;
@@ -91,6 +94,7 @@ template <int NC, int ND>
 __global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe_lds(
     Links<double, NC, false> f, Geom g, int pass, int nhits, double beta_nc, double delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  f.nd = ND;
   __shared__ unsigned long long red[kMetroBlock / 64];
   __shared__ cx<double> stage[kMetroBlock / 64][3 * NC * NC * 64];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -266,19 +270,19 @@ __device__ __forceinline__ unsigned long long ws_pass(const Links<R, NC, COMP> &
         ab = mul_ad(f.load(s_xp, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), sg));
       if (keep && nu == sg + 1) *keep = ab;
       if constexpr (DEFECT)
-        add_to(acc, mul_ad(ab, ld(-1, 0, -1, 0, nu)));
+        acc_mul_ad(acc, ab, ld(-1, 0, -1, 0, nu));
       else
-        add_to(acc, mul_ad(ab, f.load(s, nu)));
+        acc_mul_ad(acc, ab, f.load(s, nu));
     };
     auto lower = [&](int j) {
       if (!((lmask >> j) & 1u)) return;
       const int nu = j < sg ? j : j + 1;
       if constexpr (DEFECT) {
-        add_to(acc, mul(mul_dd(ld(sg, 1, nu, -1, nu), ld(nu, -1, -1, 0, sg)), ld(nu, -1, -1, 0, nu)));
+        acc_mul(acc, mul_dd(ld(sg, 1, nu, -1, nu), ld(nu, -1, -1, 0, sg)), ld(nu, -1, -1, 0, nu));
       } else {
         const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
-        add_to(acc, mul(mul_dd(f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu), f.load(s_xmnu, sg)),
-                        f.load(s_xmnu, nu)));
+        acc_mul(acc, mul_dd(f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu), f.load(s_xmnu, sg)),
+                f.load(s_xmnu, nu));
       }
     };
     if constexpr (ILV) {
@@ -326,7 +330,7 @@ __device__ __forceinline__ unsigned long long ws_pass(const Links<R, NC, COMP> &
 #pragma unroll
               for (int k = 0; k < NC; ++k) Up.m[i][k] = scale(cp, Up.m[i][k]);
           }
-          add_to(S, mul_ad(lds_get<R, NC>(bA), Up));
+          acc_mul_ad(S, lds_get<R, NC>(bA), Up);
         } else {
           add_to(S, lds_get<R, NC>(lc.bu));
         }
@@ -386,6 +390,7 @@ template <typename R, int NC, bool COMP, int ND, bool ILV>
 __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  f.nd = ND;
   __shared__ cx<R> stage[4 * 4 * NC * NC * 64];
   __shared__ unsigned long long red[kWsBlock / 64];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -456,11 +461,11 @@ __device__ __forceinline__ Mat<R, NC> staple_part(const Links<R, NC, COMP> &f, c
     if (upper) {
       const Mat<R, NC> ab = mul_ad(f.load(idx_shift<ND>(g, x, mu, 1, -1, 0), nu),
                                    f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
-      add_to(t, mul_ad(ab, f.load(dev_index<ND>(g, x), nu)));
+      acc_mul_ad(t, ab, f.load(dev_index<ND>(g, x), nu));
     } else {
       const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
       const Mat<R, NC> ab = mul_dd(f.load(idx_shift<ND>(g, x, mu, 1, nu, -1), nu), f.load(s_xmnu, mu));
-      add_to(t, mul(ab, f.load(s_xmnu, nu)));
+      acc_mul(t, ab, f.load(s_xmnu, nu));
     }
   }
   return t;
@@ -481,6 +486,7 @@ template <typename R, int NC, bool COMP, int ND, int OCC, int LANES>
 __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_group(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+  f.nd = ND;
   static_assert(LANES == 2 || LANES == 4, "two or four lanes per site");
   __shared__ unsigned long long red[kMetroBlock / 64];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
