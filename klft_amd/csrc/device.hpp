@@ -224,24 +224,39 @@ __device__ __forceinline__ void acc_mul_ad(Mat<R, N>& t, const Mat<R, N>& a, con
       t.m[i][j] = v;
     }
 }
-// Re Tr(a * b), two FMAs per term
+// Re Tr(a * b), two FMAs per term, one accumulator per row (independent FMA
+// chains instead of one chain of 2 N^2 dependent FMAs), rows summed in order
 template <typename R, int N>
 __device__ __forceinline__ R retr_mul(const Mat<R, N>& a, const Mat<R, N>& b) {
-  R t = R(0);
+  R r[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < N; ++i) {
+    R t = a.m[i][0].re * b.m[0][i].re;
+    t = fmar(-a.m[i][0].im, b.m[0][i].im, t);
 #pragma unroll
-    for (int k = 0; k < N; ++k) t = fmar(-a.m[i][k].im, b.m[k][i].im, fmar(a.m[i][k].re, b.m[k][i].re, t));
+    for (int k = 1; k < N; ++k) t = fmar(-a.m[i][k].im, b.m[k][i].im, fmar(a.m[i][k].re, b.m[k][i].re, t));
+    r[i] = t;
+  }
+  R t = r[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) t += r[i];
   return t;
 }
-// Re Tr(a * b^dagger) = sum_ij Re(a_ij conj(b_ij)), summed c1 outer, c2 inner
+// Re Tr(a * b^dagger) = sum_ij Re(a_ij conj(b_ij)), rows (c1) summed in order
 template <typename R, int N>
 __device__ __forceinline__ R retr_mul_ad(const Mat<R, N>& a, const Mat<R, N>& b) {
-  R t = R(0);
+  R r[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < N; ++i) {
+    R t = a.m[i][0].re * b.m[i][0].re;
+    t = fmar(a.m[i][0].im, b.m[i][0].im, t);
 #pragma unroll
-    for (int j = 0; j < N; ++j) t = fmar(a.m[i][j].im, b.m[i][j].im, fmar(a.m[i][j].re, b.m[i][j].re, t));
+    for (int j = 1; j < N; ++j) t = fmar(a.m[i][j].im, b.m[i][j].im, fmar(a.m[i][j].re, b.m[i][j].re, t));
+    r[i] = t;
+  }
+  R t = r[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) t += r[i];
   return t;
 }
 template <typename R, int N>
@@ -685,9 +700,13 @@ __device__ __forceinline__ double plaq_at(const Links<R, NC, COMP>& f, const Geo
     const long long sm = idx_shift<ND>(g, x, mu, 1, -1, 0);
 #pragma unroll 1
     for (int nu = mu + 1; nu < ND; ++nu) {
-      const Mat<R, NC> lmu = mul(f.load(s, mu), f.load(sm, nu));
-      const Mat<R, NC> lnu = mul(f.load(s, nu), f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
-      p += (double)retr_mul_ad(lmu, lnu);
+      // the plane's four loads are issued together before any product (the
+      // scheduler otherwise interleaves them with the FMA chains and waits
+      // on each in turn)
+      const Mat<R, NC> a = f.load(s, mu), b = f.load(sm, nu);
+      const Mat<R, NC> c = f.load(s, nu), d = f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu);
+      __builtin_amdgcn_sched_barrier(0);
+      p += (double)retr_mul_ad(mul(a, b), mul(c, d));
     }
   }
   return p;
@@ -701,14 +720,19 @@ __device__ __forceinline__ double plaq_at_full(const Links<R, NC, COMP>& f, cons
 #pragma unroll
   for (int d = 0; d < ND; ++d) U[d] = f.load(n.s, d);
   double p = 0.0;
+  // the planes of one mu form a group whose loads are issued together
 #pragma unroll
-  for (int mu = 0; mu < ND - 1; ++mu)
+  for (int mu = 0; mu < ND - 1; ++mu) {
+    Mat<R, NC> a[ND], b[ND];
 #pragma unroll
     for (int nu = mu + 1; nu < ND; ++nu) {
-      const Mat<R, NC> lmu = mul(U[mu], f.load(n.s + n.dp[mu], nu));
-      const Mat<R, NC> lnu = mul(U[nu], f.load(n.s + n.dp[nu], mu));
-      p += (double)retr_mul_ad(lmu, lnu);
+      a[nu] = f.load(n.s + n.dp[mu], nu);
+      b[nu] = f.load(n.s + n.dp[nu], mu);
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int nu = mu + 1; nu < ND; ++nu) p += (double)retr_mul_ad(mul(U[mu], a[nu]), mul(U[nu], b[nu]));
+  }
   return p;
 }
 
