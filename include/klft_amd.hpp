@@ -69,6 +69,9 @@ struct MetropolisParams {
   index_t nHits = 10, nSweep = 1000, seed = 1234;
   size_t Nd = 4, Nc = 2;
   real_t beta = 1.0, delta = 0.1;
+  // SU(3) proposal law (no counterpart in the reference): KLFT_PROPOSAL_REFERENCE
+  // or, from the optional input key "proposal: symmetric", KLFT_PROPOSAL_SYMMETRIC
+  int proposal = KLFT_PROPOSAL_REFERENCE;
   void print() const {
     if (KLFT_VERBOSITY > 0) {
       printf("Metropolis Parameters:\n");
@@ -87,6 +90,7 @@ struct MetropolisParams {
       printf("Wilson Action Parameters:\n");
       printf("beta: %.4f\n", beta);
       printf("delta: %.4f\n", delta);
+      if (proposal == KLFT_PROPOSAL_SYMMETRIC) printf("proposal: symmetric\n");
     }
   }
 };
@@ -467,6 +471,15 @@ inline bool parseInputFile(const std::string &filename, MetropolisParams &mp) {
       mp.Nc = m["Nc"].as<size_t>(2);
       mp.beta = m["beta"].as<double>(1.0);
       mp.delta = m["delta"].as<double>(0.1);
+      const std::string law = m["proposal"].as<std::string>("reference");
+      if (law == "symmetric") {
+        mp.proposal = KLFT_PROPOSAL_SYMMETRIC;
+      } else if (law == "reference") {
+        mp.proposal = KLFT_PROPOSAL_REFERENCE;
+      } else {
+        printf("Error parsing input file: unknown proposal %s\n", law.c_str());
+        return false;
+      }
     } else {
       printf("Error: MetropolisParams not found in input file\n");
       return false;

@@ -125,6 +125,16 @@ typedef struct {
  * naccepted (optional) = number of accepted hits. */
 int klft_metropolis_sweep(klft_field_t f, const klft_metropolis_params *p, uint64_t sweep,
                           double *acc_rate, int64_t *naccepted);
+/* SU(3) Metropolis proposal law of a field.  KLFT_PROPOSAL_REFERENCE (the
+ * default) is randSUN<3>(delta) (SUN.hpp:192-251), whose rows are normalised
+ * uniform(0, delta) vectors: delta cancels and the law is asymmetric, so the
+ * chain does not satisfy detailed balance.  KLFT_PROPOSAL_SYMMETRIC has no
+ * counterpart in the reference: r = exp(i delta sum_a h_a lambda_a),
+ * h_a ~ U(-1, 1), a symmetric law (r and r^dagger equally likely) whose step
+ * size delta sets.  U(1) and SU(2) proposals are symmetric either way. */
+#define KLFT_PROPOSAL_REFERENCE 0
+#define KLFT_PROPOSAL_SYMMETRIC 1
+int klft_field_set_proposal(klft_field_t f, int proposal);
 /* n sweeps sweep0 .. sweep0+n-1 with no host synchronisation in between;
  * acc_rates (optional) receives n values. */
 int klft_metropolis_run(klft_field_t f, const klft_metropolis_params *p, uint64_t sweep0,

@@ -25,6 +25,7 @@ STORAGE_F64 = 0
 STORAGE_F32 = 1
 STORAGE_F32_ROWS = 2
 STORAGES = {"f64": STORAGE_F64, "f32": STORAGE_F32, "f32_rows": STORAGE_F32_ROWS}
+PROPOSALS = {"reference": 0, "symmetric": 1}
 
 
 class KlftError(RuntimeError):
@@ -113,6 +114,7 @@ _SIGS = {
     "klft_metropolis_sweep": (C.c_int, [_p, C.POINTER(MetropolisParamsC), C.c_uint64, _dp, _i64p]),
     "klft_metropolis_run": (C.c_int, [_p, C.POINTER(MetropolisParamsC), C.c_uint64, C.c_int, _dp]),
     "klft_metropolis_schedule": (C.c_int, [_p, _i32p]),
+    "klft_field_set_proposal": (C.c_int, [_p, C.c_int]),
     "klft_hmc_create": (C.c_int, [_p, C.POINTER(HmcParamsC), C.POINTER(_p)]),
     "klft_hmc_destroy": (C.c_int, [_p]),
     "klft_hmc_refresh_momenta": (C.c_int, [_p, C.c_uint64]),
@@ -257,6 +259,13 @@ class GaugeField:
 
     def reunitarize(self):
         _check(lib.klft_field_reunitarize(self.h))
+        return self
+
+    def set_proposal(self, law: str):
+        """SU(3) Metropolis proposal: "reference" (randSUN<3>, SUN.hpp:192-251)
+        or "symmetric" (exp(i delta h.lambda), h_a ~ U(-1, 1); no counterpart
+        in the reference)."""
+        _check(lib.klft_field_set_proposal(self.h, PROPOSALS[law]))
         return self
 
 

@@ -1070,6 +1070,31 @@ __device__ __forceinline__ Mat<R, 3> exp_iq(const R* v, Mat<R, 3>*) {
   return e;
 }
 
+// The opt-in symmetric SU(3) proposal (no counterpart in the reference, whose
+// randSUN<3> is asymmetric, SUN.hpp:192-251): r = exp(i delta sum_a h_a
+// lambda_a), h_a ~ U(-1, 1) for a = 0..7 in stream order (tag 9), then the
+// accept uniform.  h -> -h maps r to r^dagger with the same density, so the
+// proposal is symmetric and the Metropolis chain satisfies detailed balance.
+// U(1) and SU(2) proposals (SUN.hpp:163-191) are symmetric already.
+template <typename R, int NC, bool SYM>
+__device__ __forceinline__ void propose_law(uint64_t seed, uint32_t site, uint32_t sweep, uint32_t hit,
+                                            uint32_t chain, uint32_t mu, R delta, Mat<R, NC>& r, R& u) {
+  if constexpr (SYM && NC == 3) {
+    constexpr int NB = (9 * DrawWords<R>::value + 3) / 4;
+    const klft_rng::Stream st(seed, klft_rng::TAG_METROPOLIS_SYM, site, sweep, hit, chain, mu);
+    uint32_t w[4 * NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) st.block_words((uint32_t)j, w + 4 * j);
+    R v[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) v[a] = delta * draw_at<R>(w, a, R(-1), R(1));
+    r = exp_iq(v, (Mat<R, 3>*)nullptr);
+    u = u01_at<R>(w, 8);
+  } else {
+    propose<R, NC>(seed, site, sweep, hit, chain, mu, delta, r, u);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // reductions (wave64 shuffles -> LDS -> per-block partial)
 // ---------------------------------------------------------------------------

@@ -13,12 +13,15 @@ template <size_t Nd, size_t Nc>
 int run(const klft::MetropolisParams &mp, klft::GaugeObservableParams &gp, const klft::RNGType &rng) {
   if constexpr (Nd == 4) {
     klft::deviceGaugeField<4, Nc> g(mp.L0, mp.L1, mp.L2, mp.L3, klft::identitySUN<Nc>());
+    klft::check(klft_field_set_proposal(g.h, mp.proposal));
     return klft::run_metropolis<4, Nc>(g, mp, gp, rng);
   } else if constexpr (Nd == 3) {
     klft::deviceGaugeField<3, Nc> g(mp.L0, mp.L1, mp.L2, klft::identitySUN<Nc>());
+    klft::check(klft_field_set_proposal(g.h, mp.proposal));
     return klft::run_metropolis<3, Nc>(g, mp, gp, rng);
   } else {
     klft::deviceGaugeField<2, Nc> g(mp.L0, mp.L1, klft::identitySUN<Nc>());
+    klft::check(klft_field_set_proposal(g.h, mp.proposal));
     return klft::run_metropolis<2, Nc>(g, mp, gp, rng);
   }
 }

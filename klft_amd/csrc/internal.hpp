@@ -32,6 +32,7 @@ struct klft_field_s {
   // kept until the field is destroyed)
   unsigned long long *run_counters = nullptr;
   int run_capacity = 0;
+  int proposal = KLFT_PROPOSAL_REFERENCE;  // SU(3) Metropolis proposal law
 };
 
 namespace klft_internal {

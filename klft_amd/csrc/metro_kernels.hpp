@@ -219,8 +219,8 @@ __device__ __forceinline__ Mat<R, NC> adjoint(const Mat<R, NC> &a) {
 // product, the own link of the next direction, and the link being updated:
 // 4 x 4 x Nc^2 x 64 complex.  With DEFECT every link read is scaled by c on
 // the defect and the updated link's own factor multiplies dS, as in
-// metro_site.
-template <typename R, int NC, bool COMP, int ND, bool ILV, bool DEFECT>
+// metro_site.  SYM draws the opt-in symmetric SU(3) proposal (propose_law).
+template <typename R, int NC, bool COMP, int ND, bool ILV, bool DEFECT, bool SYM = false>
 __device__ __forceinline__ unsigned long long ws_pass(const Links<R, NC, COMP> &f, const Geom &g, int pass,
                                                       long long t, int nhits, R beta_nc, R delta, uint64_t seed,
                                                       uint32_t sweep, uint32_t chain, int *visit, const Defect &d,
@@ -343,7 +343,7 @@ __device__ __forceinline__ unsigned long long ws_pass(const Links<R, NC, COMP> &
         for (int hit = 0; hit < nhits; ++hit) {
           Mat<R, NC> r;
           R u;
-          propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
+          propose_law<R, NC, SYM>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
           lc.step(r, u, beta_nc, cl, n);
         }
         lc.apply_pending();
@@ -386,7 +386,7 @@ __device__ __forceinline__ void ws_count(unsigned long long n, unsigned long lon
   }
 }
 
-template <typename R, int NC, bool COMP, int ND, bool ILV>
+template <typename R, int NC, bool COMP, int ND, bool ILV, bool SYM = false>
 __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
@@ -395,8 +395,9 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
   __shared__ unsigned long long red[kWsBlock / 64];
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   const long long t = (long long)lb * kWsSites + (long long)((threadIdx.x >> 6) & 3) * 64 + (threadIdx.x & 63);
-  const unsigned long long n = ws_pass<R, NC, COMP, ND, ILV, false>(f, g, pass, t, nhits, beta_nc, delta, seed,
-                                                                    sweep, chain, visit, Defect{}, R(1), stage);
+  const unsigned long long n = ws_pass<R, NC, COMP, ND, ILV, false, SYM>(f, g, pass, t, nhits, beta_nc, delta,
+                                                                         seed, sweep, chain, visit, Defect{}, R(1),
+                                                                         stage);
   ws_count(n, red, nacc);
 }
 
@@ -543,7 +544,8 @@ __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_group(
 // next staple loaded during the hits), 8 (as 7, FP64 links staged in LDS)
 // 9, 10 (pair per site, occupancy 2, 4), 11 (one lane, own links kept
 // in registers), 12 (site waves fed by staple waves) or 13 (as 12, each
-// direction's upper and lower staple terms back to back)
+// direction's upper and lower staple terms back to back); 14, 15: 12, 13
+// with the symmetric SU(3) proposal
 template <typename RR, bool CC>
 int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1,
                         const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
@@ -560,7 +562,21 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
     const unsigned nb = blocks_for(f->g.Vh * (quad ? 4 : (pair ? 2 : 1)), kMetroBlock);
     for (int pass = p0; pass < p1; ++pass) {
       ProfScope prof("metropolis_pass");
-      if (variant == 12)
+      if (variant == 14 || variant == 15) {
+        // the symmetric SU(3) proposal: site waves, terms as in 12 / 13
+        if constexpr (NC == 3) {
+          if (variant == 14)
+            hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, false, true>), dim3(blocks_for(f->g.Vh, kWsSites)),
+                               dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta,
+                               p->seed, (uint32_t)sweep, chain, counter, visit);
+          else
+            hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, true, true>), dim3(blocks_for(f->g.Vh, kWsSites)),
+                               dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta,
+                               p->seed, (uint32_t)sweep, chain, counter, visit);
+        } else {
+          return set_error(KLFT_E_ARG, "the symmetric proposal variants are SU(3) only");
+        }
+      } else if (variant == 12)
         hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, false>), dim3(blocks_for(f->g.Vh, kWsSites)),
                            dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
                            (uint32_t)sweep, chain, counter, visit);

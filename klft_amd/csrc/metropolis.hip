@@ -78,6 +78,8 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool tim
   // order for FP64 links and the upper-terms-first order for FP32 links
   // (the interleaved order loses at 48^4 in FP32).
   const int dflt = f->storage == KLFT_STORAGE_F64 ? 13 : 12;
+  // the symmetric SU(3) proposal runs in the site-wave kernels only
+  if (f->proposal == KLFT_PROPOSAL_SYMMETRIC && f->nc == 3) return dflt + 2;
   // the tuning key carries nHits: the best variant depends on the hit count
   const std::string kname = "metropolis_nhits_" + std::to_string(p->nhits);
   if (!timed) return chosen_variant(kname.c_str(), f, kMetroVariants, dflt);
@@ -194,4 +196,13 @@ extern "C" int klft_metropolis_schedule(klft_field_t f, int32_t *visit) {
   }
   (void)hipFree(dv);
   return rc;
+}
+
+extern "C" int klft_field_set_proposal(klft_field_t f, int proposal) {
+  int rc = ki::check_field(f);
+  if (rc) return rc;
+  if (proposal != KLFT_PROPOSAL_REFERENCE && proposal != KLFT_PROPOSAL_SYMMETRIC)
+    return ki::set_error(KLFT_E_ARG, "unknown proposal law %d", proposal);
+  f->proposal = proposal;
+  return KLFT_OK;
 }

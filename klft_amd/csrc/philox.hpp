@@ -37,7 +37,8 @@ enum : uint32_t {
   TAG_RANDSUN = 5,
   TAG_COMPLEX = 6,
   TAG_ACCEPT = 7,
-  TAG_SWAP = 8
+  TAG_SWAP = 8,
+  TAG_METROPOLIS_SYM = 9
 };
 
 // a ^ b ^ c; on gfx950 one v_bitop3_b32 (truth table 0x96) instead of two XORs

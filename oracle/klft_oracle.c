@@ -736,9 +736,20 @@ void orc_field_random_complex(int nd, int nc, const int *dims, uint64_t seed, do
 /* ------------------------------------------------------------------------ */
 /* Metropolis (Metropolis.hpp:72-178)                                       */
 /* ------------------------------------------------------------------------ */
+/* The opt-in symmetric SU(3) proposal (no counterpart in the reference,
+ * whose randSUN<3> is asymmetric, SUN.hpp:192-251): r = exp(i delta sum_a
+ * h_a lambda_a) with h_a = drand(-1, 1), a = 0..7, drawn from the stream
+ * (tag 9); the accept uniform follows.  h -> -h maps r to r^dagger with the
+ * same density, so the proposal is symmetric. */
+static void rand_sym3(stream *s, double delta, sun *r) {
+  double v[8];
+  for (int a = 0; a < 8; ++a) v[a] = delta * drand(s, -1.0, 1.0);
+  *r = exp_iq(3, v);
+}
+
 static double metropolis_impl(const lat *l, double *U, const double *C, int nhits, double beta,
                               double delta, uint64_t seed, uint32_t sweep, uint32_t chain,
-                              long *naccepted, int *visit) {
+                              long *naccepted, int *visit, int sym) {
   int nd = l->nd, nc = l->nc;
   long H[4] = {1, 1, 1, 1}, Vh = 1;
   for (int d = 0; d < nd; ++d) {
@@ -766,9 +777,14 @@ static double metropolis_impl(const lat *l, double *U, const double *C, int nhit
         double cl = C ? C[(size_t)site * nd + mu] : 1.0;
         for (int hit = 0; hit < nhits; ++hit) {
           stream st;
-          stream_init(&st, seed, 1, (uint32_t)site, sweep, (uint32_t)hit, chain, (uint32_t)mu);
           sun rr;
-          rand_sun(nc, &st, delta, &rr);
+          if (sym && nc == 3) {
+            stream_init(&st, seed, 9, (uint32_t)site, sweep, (uint32_t)hit, chain, (uint32_t)mu);
+            rand_sym3(&st, delta, &rr);
+          } else {
+            stream_init(&st, seed, 1, (uint32_t)site, sweep, (uint32_t)hit, chain, (uint32_t)mu);
+            rand_sun(nc, &st, delta, &rr);
+          }
           sun Uold = link(l, U, site, mu);
           sun Unew = mm(nc, &Uold, &rr);
           double dS;
@@ -813,7 +829,27 @@ double orc_metropolis_sweep(int nd, int nc, const int *dims, double *U, int nhit
                             double delta, uint64_t seed, uint32_t sweep, long *naccepted,
                             int *visit) {
   lat l = mklat(nd, nc, dims);
-  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, visit);
+  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, visit, 0);
+}
+
+double orc_metropolis_sweep_sym(int nd, int nc, const int *dims, double *U, int nhits, double beta,
+                                double delta, uint64_t seed, uint32_t sweep, long *naccepted) {
+  lat l = mklat(nd, nc, dims);
+  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, NULL, 1);
+}
+
+int orc_rand_sym3(double delta, uint64_t seed, uint32_t sweep, uint32_t site, uint32_t hit,
+                  uint32_t mu, double *out) {
+  stream s;
+  stream_init(&s, seed, 9, site, sweep, hit, 0, mu);
+  sun r;
+  rand_sym3(&s, delta, &r);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      out[2 * (i * 3 + j)] = r.m[i][j].re;
+      out[2 * (i * 3 + j) + 1] = r.m[i][j].im;
+    }
+  return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1006,5 +1042,5 @@ double orc_metropolis_sweep_defect(int nd, int nc, const int *dims, double *U, c
                                    int nhits, double beta, double delta, uint64_t seed,
                                    uint32_t sweep, uint32_t chain, long *naccepted) {
   lat l = mklat(nd, nc, dims);
-  return metropolis_impl(&l, U, C, nhits, beta, delta, seed, sweep, chain, naccepted, NULL);
+  return metropolis_impl(&l, U, C, nhits, beta, delta, seed, sweep, chain, naccepted, NULL, 0);
 }

@@ -75,6 +75,12 @@ double orc_metropolis_sweep(int nd, int nc, const int *dims, double *U,
                             int nhits, double beta, double delta,
                             uint64_t seed, uint32_t sweep, long *naccepted,
                             int *visit);
+/* the same sweep with the opt-in symmetric SU(3) proposal (tag 9):
+ * r = exp(i delta sum_a h_a lambda_a), h_a = drand(-1, 1) */
+double orc_metropolis_sweep_sym(int nd, int nc, const int *dims, double *U, int nhits, double beta,
+                                double delta, uint64_t seed, uint32_t sweep, long *naccepted);
+int orc_rand_sym3(double delta, uint64_t seed, uint32_t sweep, uint32_t site, uint32_t hit,
+                  uint32_t mu, double *out);
 
 /* --- HMC ------------------------------------------------------------------- */
 int orc_ngen(int nc);

@@ -159,6 +159,26 @@ def wilson_temporal(U, nd, nc, dims, pairs, normalize=True):
     return out[:len(pairs)]
 
 
+L.orc_metropolis_sweep_sym.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_double,
+                                       C.c_uint64, C.c_uint32, C.POINTER(C.c_long)]
+L.orc_metropolis_sweep_sym.restype = C.c_double
+L.orc_rand_sym3.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _dp]
+
+
+def metropolis_sweep_sym(U, nd, nc, dims, nhits, beta, delta, seed, sweep):
+    """Metropolis sweep with the opt-in symmetric SU(3) proposal, in place on U."""
+    assert U.flags["C_CONTIGUOUS"] and U.dtype == np.complex128
+    n = C.c_long()
+    acc = L.orc_metropolis_sweep_sym(nd, nc, _i(dims), _d(U), nhits, beta, delta, seed, sweep, C.byref(n))
+    return acc, n.value
+
+
+def rand_sym3(delta, seed, sweep, site, hit, mu):
+    out = np.empty((3, 3), dtype=np.complex128)
+    L.orc_rand_sym3(delta, seed, sweep, site, hit, mu, _d(out))
+    return out
+
+
 def metropolis_sweep(U, nd, nc, dims, nhits, beta, delta, seed, sweep, visit=False):
     """In place on U (must be C-contiguous complex128). Returns (acc, naccepted[, visit])."""
     assert U.flags["C_CONTIGUOUS"] and U.dtype == np.complex128

@@ -109,6 +109,32 @@ def test_metropolis_driver_files_match_oracle(tmp_path, nc, beta):
     assert (tmp_path / "plaquette.out").read_text().count("# step, plaquette") == 2
 
 
+def test_metropolis_driver_symmetric_proposal(tmp_path):
+    """`proposal: symmetric` in MetropolisParams (an input key the reference
+    does not have) selects the symmetric SU(3) law; the plaquette file
+    matches the oracle chain with that law, and an unknown law is an error."""
+    nsweep = 21
+    text = INPUT.format(nc=3, nsweep=nsweep, beta=5.7).replace("delta: 0.1", "delta: 0.3\n  proposal: symmetric")
+    (tmp_path / "input.yaml").write_text(text)
+    r = subprocess.run([str(BIN / "metropolis"), "-f", "input.yaml"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dims = (8, 8, 8, 8)
+    U = orc.identity(4, 3, dims)
+    plaq = []
+    for step in range(nsweep):
+        orc.metropolis_sweep_sym(U, 4, 3, dims, 10, 5.7, 0.3, 32091, step)
+        if step % 10 == 0 and step != 0:
+            plaq.append(orc.plaquette(U, 4, 3, dims))
+    _, got = rows(tmp_path / "plaquette.out")
+    np.testing.assert_allclose([g[1] for g in got], plaq, rtol=2e-6)
+    assert plaq[-1] < 0.99  # the symmetric law moves the cold start
+    (tmp_path / "input.yaml").write_text(text.replace("proposal: symmetric", "proposal: sideways"))
+    r = subprocess.run([str(BIN / "metropolis"), "-f", "input.yaml"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "unknown proposal" in r.stdout
+
+
 def test_write_to_file_disabled(tmp_path):
     (tmp_path / "input.yaml").write_text(
         INPUT.format(nc=1, nsweep=12, beta=2.0).replace("write_to_file: true", "write_to_file: false"))
