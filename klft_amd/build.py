@@ -23,7 +23,8 @@ OBJ = HERE / "_build"
 LIB = HERE / "libklft_hip.so"
 INCLUDE = HERE.parent / "include"
 
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", f"{ROCM}/bin/hipcc")
 ARCH = "gfx950"
 
 # Packed FP32 instructions are switched off for every device pass.
@@ -74,7 +75,10 @@ def build(jobs: int | None = None, tools: bool = True) -> Path:
         for f in futs:
             objs.append(f.result())
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)])
+    # roctx ranges (the reference's kernel labels) come from the ROCm
+    # profiler SDK's marker library
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs),
+          f"-L{ROCM}/lib", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, LIB)
     if tools:
         for name, src in TOOLS.items():

@@ -626,7 +626,7 @@ static int launch_plaquette(klft_field_t f, const RedBuf &rb, double *dev_per_si
       const int nsegs = (f->g.L[0] + S - 1) / S;
       const unsigned nb = (unsigned)(nsegs * ((Vs + 255) / 256));
       if ((long long)nb > rb.cap) return set_error(KLFT_E_ARG, "lattice too large for reduction");
-      ProfScope prof("plaquette");
+      ProfScope prof("plaquette", "GaugePlaquette_GaugeField");
       if (variant == 4)
         hipLaunchKernelGGL((k_plaquette_walk<R, NC, COMP, ND, true>), dim3(nb), dim3(256), 0, stream(),
                            links_of<R, NC, COMP>(f), f->g, S, Vs, rb.partials, rb.ticket, rb.out,
@@ -641,7 +641,7 @@ static int launch_plaquette(klft_field_t f, const RedBuf &rb, double *dev_per_si
     const unsigned nb = blocks_for(chunk_threads<ND>(f->g), 256);
     if ((long long)nb > rb.cap) return set_error(KLFT_E_ARG, "lattice too large for reduction");
     if (variant == 3) {
-      ProfScope prof("plaquette");
+      ProfScope prof("plaquette", "GaugePlaquette_GaugeField");
       hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, 1, true>), dim3(nb), dim3(256), 0, stream(),
                          links_of<R, NC, COMP>(f), f->g, rb.partials, rb.ticket, rb.out, dev_per_site);
       KLFT_HIP(hipGetLastError());
@@ -649,7 +649,7 @@ static int launch_plaquette(klft_field_t f, const RedBuf &rb, double *dev_per_si
     }
     return with_occ(variant, [&](auto occ) -> int {
       constexpr int OCC = decltype(occ)::value;
-      ProfScope prof("plaquette");
+      ProfScope prof("plaquette", "GaugePlaquette_GaugeField");
       hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, OCC>), dim3(nb), dim3(256), 0, stream(),
                          links_of<R, NC, COMP>(f), f->g, rb.partials, rb.ticket, rb.out,
                          dev_per_site);
@@ -814,7 +814,7 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
       KLFT_UNPACK(rt, nct, ct, ndt);
       const unsigned nb = (unsigned)link_blocks<ND>(f->g);
       if (variant == 3) {
-        ki::ProfScope prof("staple_field");
+        ki::ProfScope prof("staple_field", "stapleField_GaugeField");
         hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND, 1, false>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
                            ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
         KLFT_HIP(hipGetLastError());
@@ -822,7 +822,7 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
       }
       return ki::with_occ(variant, [&](auto occ) -> int {
         constexpr int OCC = decltype(occ)::value;
-        ki::ProfScope prof("staple_field");
+        ki::ProfScope prof("staple_field", "stapleField_GaugeField");
         hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND, OCC>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
                            ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
         KLFT_HIP(hipGetLastError());
@@ -1000,7 +1000,7 @@ int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, 
   }
   int rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
-    ki::ProfScope prof("wilson");
+    ki::ProfScope prof("wilson", "WilsonLoop_GaugeField_WLoop_munu");
     hipLaunchKernelGGL((k_wilson<R, NC, COMP, ND>), dim3(nbx, nplanes), dim3(block),
                        block * sizeof(Mat<R, NC>), ki::stream(), ki::links_of<R, NC, COMP>(f), f->g,
                        nplanes > 1 ? -1 : mu0, nu, C, nL, dLs, Tmax, dpart, dps, ps_li, ps_T);

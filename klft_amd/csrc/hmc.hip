@@ -293,7 +293,7 @@ int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int vari
     const R coef = (R)(h->p.beta / (double)NC);
     long long nb = 0;
     {
-      ki::ProfScope prof("force");
+      ki::ProfScope prof("force", "derivative+update_momentum");
       if (variant == 3) {
         nb = ki::blocks_for(chunk_threads<ND>(f->g), 256);
         if (nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
@@ -334,7 +334,7 @@ int enqueue_gauge(klft_hmc_t h, double dtau) {
   klft_field_t f = h->f;
   return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
-    ki::ProfScope prof("update_gauge");
+    ki::ProfScope prof("update_gauge", "update_gauge");
     hipLaunchKernelGGL((k_update_gauge<R, NC, COMP, ND>), dim3(ki::blocks_for(f->g.V, 256)), dim3(256),
                        0, ki::stream(), ki::links_of<R, NC, COMP>(f), (const R *)h->P, f->g, (R)dtau);
     KLFT_HIP(hipGetLastError());
@@ -371,7 +371,7 @@ int launch_md_step(klft_hmc_t h, void *src, void *dst, double dtp, double dtu, i
     ld.p = reinterpret_cast<cx<R> *>(dst);
     const unsigned nb = (unsigned)link_blocks<ND>(f->g);
     const R coef = (R)(h->p.beta / (double)NC);
-    ki::ProfScope prof("md_step");
+    ki::ProfScope prof("md_step", "derivative+update_momentum+update_gauge");
     if (variant == 1)
       hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, false>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,
                          (R *)h->P, f->g, coef, (R)dtp, (R)dtu);

@@ -10,6 +10,8 @@
 #include <string>
 #include <type_traits>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/klft_hip.h"
 #include "device.hpp"
 
@@ -70,12 +72,16 @@ using ic = std::integral_constant<int, V>;
 
 // ---- kernel timing (tuner.hip): HIP events on the library stream around
 // launches, enabled with klft_profile(1); read back with klft_profile_read ----
+// Each launch also sits in a roctx range named with the reference's own
+// kernel label (e.g. "sweep_Metropolis_GaugeField_sublat_3",
+// Metropolis.hpp:161), which rocprofv3 --marker-trace records.
 bool prof_enabled();
 void prof_record(const char *name, hipEvent_t start, hipEvent_t stop);
 struct ProfScope {
   const char *name;
   hipEvent_t a = nullptr, b = nullptr;
-  explicit ProfScope(const char *n) : name(n) {
+  ProfScope(const char *n, const char *label) : name(n) {
+    roctxRangePushA(label);
     if (prof_enabled() && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
       (void)hipEventRecord(a, stream());
     else
@@ -86,6 +92,7 @@ struct ProfScope {
       (void)hipEventRecord(b, stream());
       prof_record(name, a, b);
     }
+    roctxRangePop();
   }
 };
 

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <type_traits>
 
 #include "internal.hpp"
@@ -561,7 +562,9 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
     const bool pair = variant == 9 || variant == 10;
     const unsigned nb = blocks_for(f->g.Vh * (quad ? 4 : (pair ? 2 : 1)), kMetroBlock);
     for (int pass = p0; pass < p1; ++pass) {
-      ProfScope prof("metropolis_pass");
+      char label[48];
+      snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pass);  // Metropolis.hpp:161
+      ProfScope prof("metropolis_pass", label);
       if (variant == 14 || variant == 15) {
         // the symmetric SU(3) proposal: site waves, terms as in 12 / 13
         if constexpr (NC == 3) {

@@ -38,6 +38,20 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_library_marks_launches_with_the_reference_labels():
+    """Every launch sits in a roctx range named with the reference's kernel
+    label (Metropolis.hpp:161, GaugePlaquette.hpp:117, Gauge_Util.hpp:45,
+    WilsonLoop.hpp:226, HamiltonianField.hpp:25-34): the library imports the
+    roctx range calls and carries the label strings."""
+    assert LIB.exists(), "build the library first (python -m klft_amd.build)"
+    syms = subprocess.run(["nm", "-D", str(LIB)], capture_output=True, text=True, check=True).stdout
+    assert re.search(r"U roctxRangePushA", syms) and re.search(r"U roctxRangePop", syms)
+    blob = LIB.read_bytes()
+    for label in [b"sweep_Metropolis_GaugeField_sublat_%d", b"GaugePlaquette_GaugeField", b"stapleField_GaugeField",
+                  b"WilsonLoop_GaugeField_WLoop_munu", b"derivative+update_momentum+update_gauge"]:
+        assert label in blob, label
+
+
 def test_python_binding_covers_the_header():
     import klft_amd
     klft_amd.load()
