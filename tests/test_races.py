@@ -1,0 +1,84 @@
+"""Race emulation of the checkerboarded Metropolis schedule, on the CPU.
+
+Within one sublattice pass every thread writes the links of its own site and
+reads its staple neighbourhood (GaugeField.hpp:187-237): U_nu(x+mu),
+U_mu(x+nu), U_nu(x), U_nu(x+mu-nu), U_mu(x-nu), U_nu(x-nu).  The pass is
+race free when no link read on behalf of one site is written by another
+site of the same pass (the reference relies on this without checking it,
+Metropolis.hpp:127-163).  The sets are built from the schedule the oracle
+records (the pass that updates each site, the same map the GPU kernel's own
+dump equals bit for bit in the GPU tests), so the check covers the schedule
+the product runs.  With an odd extent the periodic wrap puts neighbours in
+the same pass, which is why the library rejects odd extents.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+
+def schedule(nd, dims):
+    """pass index of every site (lexicographic), as the oracle's sweep
+    records it"""
+    U = orc.identity(nd, 1, dims)
+    _, _, visit = orc.metropolis_sweep(U, nd, 1, dims, 0, 1.0, 0.1, 1, 0, visit=True)
+    return visit.reshape(dims)
+
+
+def pass_conflicts(nd, dims, visit):
+    """number of (reader site, writer site, link) triples within one pass
+    where the writer is another site"""
+    sites = np.array(list(itertools.product(*[range(L) for L in dims])))
+    L = np.array(dims)
+
+    def lex(x):
+        return np.ravel_multi_index(tuple((x % L).T), dims)
+
+    conflicts = 0
+    for p in range(1 << nd):
+        xs = sites[visit.reshape(-1) == p]
+        writer = np.full(int(np.prod(dims)), -1)
+        writer[lex(xs)] = lex(xs)  # a site writes its own nd links
+        for mu in range(nd):
+            for nu in range(nd):
+                if nu == mu:
+                    continue
+                e_mu, e_nu = np.eye(nd, dtype=int)[mu], np.eye(nd, dtype=int)[nu]
+                for off in (e_mu, e_nu, 0 * e_mu, e_mu - e_nu, -e_nu):
+                    y = lex(xs + off)
+                    w = writer[y]
+                    conflicts += int(np.sum((w >= 0) & (w != lex(xs))))
+    return conflicts
+
+
+@pytest.mark.parametrize("nd,dims", [(2, (8, 6)), (3, (4, 6, 8)), (4, (4, 4, 6, 4)), (4, (8, 8, 8, 8))])
+def test_sublattice_passes_are_race_free(nd, dims):
+    visit = schedule(nd, dims)
+    # every site is updated exactly once per sweep, by one of the 2^nd passes
+    assert visit.min() == 0 and visit.max() == (1 << nd) - 1
+    counts = np.bincount(visit.reshape(-1), minlength=1 << nd)
+    assert np.all(counts == int(np.prod(dims)) // (1 << nd))
+    assert pass_conflicts(nd, dims, visit) == 0
+
+
+def test_odd_extent_would_race():
+    """The same parity classes on an odd extent (the reference's schedule
+    skips the last slice instead, Q6): x and x + mu with x_mu = L - 1 wrap to
+    x_mu = 0 with the same parity, so one pass reads what another of its
+    sites writes."""
+    nd, dims = 2, (5, 4)
+    x = np.array(list(itertools.product(range(5), range(4))))
+    visit = ((x[:, 0] & 1) << 1 | (x[:, 1] & 1)).reshape(dims)
+    assert pass_conflicts(nd, dims, visit) > 0
+
+
+def test_two_colour_checkerboard_would_race():
+    """An even/odd schedule (two passes by total parity) is not enough for the
+    staple: x + mu - nu has the parity of x, so 2^nd parity classes are
+    needed (oddeven_array, IndexHelper.hpp:84-93)."""
+    nd, dims = 4, (4, 4, 4, 4)
+    x = np.array(list(itertools.product(*[range(4)] * 4)))
+    visit = (x.sum(axis=1) & 1).reshape(dims)
+    assert pass_conflicts(nd, dims, visit) > 0
