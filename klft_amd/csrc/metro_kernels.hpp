@@ -286,14 +286,55 @@ __device__ __forceinline__ unsigned long long ws_pass(const Links<R, NC, COMP> &
                 f.load(s_xmnu, nu));
       }
     };
+    // upper and lower term of one nu with their link loads issued before any
+    // product: six loads in flight for FP32 links, each term's three for FP64
+    // links (six FP64 matrices would not fit the two waves per SIMD)
+    auto pair_terms = [&](int j) {
+      const int nu = j < sg ? j : j + 1;
+      const bool up = nu != defer && ((umask >> j) & 1u);
+      const bool lo = ((lmask >> j) & 1u) != 0u;
+      const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
+      Mat<R, NC> a0, a1, a2, b0, b1, b2;
+      if (up) {
+        a0 = f.load(s_xp, nu);
+        a1 = f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), sg);
+        a2 = f.load(s, nu);
+      }
+      if constexpr (sizeof(R) == 4) {
+        if (lo) {
+          b0 = f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu);
+          b1 = f.load(s_xmnu, sg);
+          b2 = f.load(s_xmnu, nu);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (up) {
+        const Mat<R, NC> ab = mul_ad(a0, a1);
+        if (keep && nu == sg + 1) *keep = ab;
+        acc_mul_ad(acc, ab, a2);
+      }
+      if constexpr (sizeof(R) == 8) {
+        if (lo) {
+          b0 = f.load(idx_shift<ND>(g, x, sg, 1, nu, -1), nu);
+          b1 = f.load(s_xmnu, sg);
+          b2 = f.load(s_xmnu, nu);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      if (lo) acc_mul(acc, mul_dd(b0, b1), b2);
+    };
     if constexpr (ILV) {
       // upper and lower term of one nu back to back: the lower term's
       // U_sg(x-nu) is the upper term's U_sg(x'+nu) of the site x' = x - 2 nu,
       // a neighbouring lane or wave, so the second read tends to hit L2
 #pragma unroll 1
       for (int j = 0; j < ND - 1; ++j) {
-        upper(j);
-        lower(j);
+        if constexpr (DEFECT) {
+          upper(j);
+          lower(j);
+        } else {
+          pair_terms(j);
+        }
       }
     } else {
 #pragma unroll 1
