@@ -11,8 +11,9 @@ roofline: the dominant kernel is the Metropolis pass.  Algorithmic bytes per
 pass = 36 * (V / 16) * B_U (32 of the 64 (parity class, direction) link groups
 read, 4 read and written; B_U = 144 / 72 / 48 bytes for FP64 / FP32 / FP32
 two-row SU(3) links), divided by the pass's average duration from HIP events
-on the library stream.  traffic: HBM bytes per pass from the committed
-rocprofv3 summary (profiles/), when present.
+on the library stream.  traffic: HBM bytes per pass and the VALU busy share
+from rocprofv3 runs of this bench command on this build (profiles/); bound
+is the roof the pass is closer to on those measurements.
 
 cpu_baseline: the FP64 CPU oracle (the build's own restatement of the
 reference's OpenMP path, not the reference) runs one sweep of the same
@@ -110,6 +111,52 @@ def hmc_leg(klft, st, a, dims):
     return out
 
 
+def batched_leg(klft, a):
+    """The batched engine at the headline's volume: 16 independent SU(3) 16^4
+    chains (1,048,576 sites, as 32^4) with FP32 links + FP64 sums, one launch
+    per sublattice pass over all chains, sweeps replayed from one captured
+    HIP graph; and the PTBC replica set (16 x SU(3) 16^4, 2^3 defect at
+    x0 = 0, c_r = 1 - r/15, nHits = 8, delta = 0.2) stepped with its swap
+    pass.  Algorithmic bytes per sweep = 36 * V_total * B_U (16 passes of
+    36 (V/16) B_U)."""
+    import numpy as np
+    out = {}
+    chains = [(3, (16, 16, 16, 16), a.beta)] * 16
+    vtot = 16 * 16 ** 4
+    for st in ["f32", "f32_rows", "f64"]:
+        b = klft.Batch(chains, st)
+        b.set_identity()
+        b.run(a.nhits, a.delta, 4242, 0, 3)
+        nsw = 20
+        t = sync_time(lambda: b.run(a.nhits, a.delta, 4242, 3, nsw))
+        alg = 36 * vtot * BYTES_PER_LINK[st]
+        out[f"chains16_su3_16^4_{st}"] = {
+            "sweeps_per_s": nsw / t, "link_updates_per_s": 4 * vtot * nsw / t, "ms_per_sweep": 1e3 * t / nsw,
+            "alg_bytes_per_sweep": alg, "achieved_GBs": alg * nsw / t / 1e9,
+            "frac_hbm": alg * nsw / t / 1e9 / PEAK_HBM_GBS}
+        b.close()
+        print(f"[bench] batched 16 x 16^4 {st}: {4 * vtot * nsw / t:.3e} lu/s", file=sys.stderr, flush=True)
+    n = 16
+    p = klft.PTBC(n, 3, (16, 16, 16, 16), 6.0, 2, [0, 0, 0], 1.0 - np.arange(n) / 15.0, "f32")
+    p.set_identity()
+    for s in range(5):
+        p.step(8, 0.2, 11, s)
+    nst = 30
+    sw = np.zeros(n - 1)
+
+    def run():
+        for s in range(nst):
+            sw[:] += p.step(8, 0.2, 11, 5 + s)
+    t = sync_time(run)
+    alg = 36 * vtot * BYTES_PER_LINK["f32"]
+    out["ptbc16_su3_16^4_f32"] = {
+        "steps_per_s": nst / t, "replica_link_updates_per_s": 4 * vtot * nst / t, "ms_per_step": 1e3 * t / nst,
+        "alg_bytes_per_step": alg, "achieved_GBs": alg * nst / t / 1e9, "frac_hbm": alg * nst / t / 1e9 / PEAK_HBM_GBS,
+        "swap_acceptance_min_max": [float(sw.min() / nst), float(sw.max() / nst)]}
+    p.close()
+    return out
+
+
 def measure_leg(klft, g, st, dims):
     V = dims[0] * dims[1] * dims[2] * dims[3]
     klft.GaugePlaquette(g)
@@ -143,8 +190,12 @@ def cpu_leg(dims, a):
                       f"{t:.1f} s, OpenMP oracle"}
 
 
-def traffic_from_profiles():
-    f = ROOT / "profiles" / "metropolis_pass_su3_32_traffic.json"
+def pass_profile():
+    """Counters of the Metropolis pass kernel measured with rocprofv3 on this
+    build (profiles/metropolis_pass_su3_32.json, made from the same bench
+    command): HBM bytes per pass from FETCH_SIZE / WRITE_SIZE scaled by the
+    copy-kernel factor, VALUBusy, and the kernel-trace average duration."""
+    f = ROOT / "profiles" / "metropolis_pass_su3_32.json"
     if f.exists():
         try:
             return json.loads(f.read_text())
@@ -423,20 +474,32 @@ def main():
             g64 = g
         else:
             g.close()
+    batched = {}
     if not a.no_extra:
         for st in precisions:
             extra[st] = hmc_leg(klft, st, a, dims)
             print(f"[bench] hmc {st}: {extra[st]['traj_per_s']:.2f} traj/s", file=sys.stderr, flush=True)
+        batched = batched_leg(klft, a)
     del g64
 
-    main_leg = legs["f64"] if "f64" in legs else legs[precisions[0]]
-    traffic = None
-    tj = traffic_from_profiles()
-    if tj and "f64" in tj:
-        traffic = tj["f64"].get("bytes_per_pass")
-    roofline = {"bound": "hbm", "kernel": "metropolis_pass (FP64 SU(3))",
+    main_st = "f64" if "f64" in legs else precisions[0]
+    main_leg = legs[main_st]
+    prof = (pass_profile() or {}).get(main_st, {})
+    traffic = prof.get("bytes_per_pass")
+    roofline = {"bound": "hbm", "kernel": f"metropolis_pass ({main_st} SU(3))",
                 "achieved": round(main_leg["achieved_GBs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(main_leg["achieved_GBs"] / PEAK_HBM_GBS, 4), "traffic": traffic}
+    if traffic:
+        # the roof that binds: HBM on the bytes the pass actually moves, or
+        # VALU issue (rocprofv3 VALUBusy of the same kernel)
+        tf = traffic / (main_leg["pass_us"] * 1e-6) / 1e12 / (PEAK_HBM_GBS / 1e3)
+        roofline["traffic_GBs"] = round(traffic / (main_leg["pass_us"] * 1e-6) / 1e9, 1)
+        roofline["traffic_frac"] = round(tf, 4)
+        vb = prof.get("valu_busy_pct")
+        if vb is not None:
+            roofline["valu_busy_frac"] = round(vb / 100.0, 4)
+            roofline["bound"] = "hbm" if tf >= vb / 100.0 else "valu"
+        roofline["profile"] = "profiles/metropolis_pass_su3_32.json"
     cpu = None
     if not a.no_cpu:
         cpu = cpu_leg(dims, a)
@@ -457,6 +520,8 @@ def main():
                 for st, d in extra.items()},
         "fp32_over_fp64": {st: legs[st]["link_updates_per_s"] / main_leg["link_updates_per_s"]
                            for st in legs},
+        "batched": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in batched.items()},
     }
     print(json.dumps(out))
 
