@@ -532,9 +532,9 @@ extern "C" int klft_field_reunitarize(klft_field_t f) {
 // ============================================================================
 // plaquette (GaugePlaquette.hpp:36-135)
 // ============================================================================
-// variants 0, 1, 2: plane by plane at >= 1, 2, 4 waves/SIMD; 3: own links
-// loaded once, all planes' loads in flight together (4, 5: k_plaquette_walk,
-// below, all loads in flight / plane by plane)
+// variants 0: plane by plane; 3: own links loaded once, all planes' loads in
+// flight together (4, 5: k_plaquette_walk, below, all loads in flight / plane
+// by plane); 1, 2 (occupancy targets of 0) never won and were removed
 template <typename R, int NC, bool COMP, int ND, int OCC, bool FULL = false>
 __global__ __launch_bounds__(256, OCC) void k_plaquette(Links<R, NC, COMP> f, Geom g, double *partials,
                                                     unsigned int *ticket, double *out,
@@ -647,23 +647,21 @@ static int launch_plaquette(klft_field_t f, const RedBuf &rb, double *dev_per_si
       KLFT_HIP(hipGetLastError());
       return KLFT_OK;
     }
-    return with_occ(variant, [&](auto occ) -> int {
-      constexpr int OCC = decltype(occ)::value;
-      ProfScope prof("plaquette", "GaugePlaquette_GaugeField");
-      hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, OCC>), dim3(nb), dim3(256), 0, stream(),
-                         links_of<R, NC, COMP>(f), f->g, rb.partials, rb.ticket, rb.out,
-                         dev_per_site);
-      KLFT_HIP(hipGetLastError());
-      return KLFT_OK;
-    });
+    if (variant != 0) return set_error(KLFT_E_ARG, "plaquette variant %d is not built", variant);
+    ProfScope prof("plaquette", "GaugePlaquette_GaugeField");
+    hipLaunchKernelGGL((k_plaquette<R, NC, COMP, ND, 1>), dim3(nb), dim3(256), 0, stream(), links_of<R, NC, COMP>(f),
+                       f->g, rb.partials, rb.ticket, rb.out, dev_per_site);
+    KLFT_HIP(hipGetLastError());
+    return KLFT_OK;
   });
 }
 // untuned defaults, the fastest at 32^4 and 48^4: threads walking dimension
 // 0, plane by plane for FP64 links, all loads in flight for FP32
 static int plaquette_default(const klft_field_s *f) { return f->storage == KLFT_STORAGE_F64 ? 5 : 4; }
+constexpr int kPlaquetteVariants = 6;
 int plaquette_raw(klft_field_t f, int slot, double *dev_per_site) {
   const RedBuf rb{f->partials, f->max_partials, f->ticket, f->result + slot};
-  const int v = pick_variant("plaquette", f, kOccVariants + 3, plaquette_default(f),
+  const int v = pick_variant("plaquette", f, kPlaquetteVariants, plaquette_default(f),
                              [&](int var) { return launch_plaquette(f, rb, nullptr, var); });
   return launch_plaquette(f, rb, dev_per_site, v);
 }
@@ -782,8 +780,8 @@ extern "C" int klft_plaquette_noalloc(klft_field_t f, int normalize, double *out
 // ============================================================================
 // staple field (Gauge_Util.hpp:27-97)
 // ============================================================================
-// variants 0, 1, 2: one-term-at-a-time staple at >= 1, 2, 4 waves/SIMD;
-// 3: full staple (all its loads in flight at once)
+// variants 0: one-term-at-a-time staple; 3: full staple (all its loads in
+// flight at once)
 template <typename R, int NC, bool COMP, int ND, int OCC, bool LOWREG = true>
 __global__ __launch_bounds__(256, OCC) void k_staple_field(Links<R, NC, COMP> f,
                                                                             Links<R, NC, false> o, Geom g) {
@@ -820,17 +818,17 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
         KLFT_HIP(hipGetLastError());
         return KLFT_OK;
       }
-      return ki::with_occ(variant, [&](auto occ) -> int {
-        constexpr int OCC = decltype(occ)::value;
-        ki::ProfScope prof("staple_field", "stapleField_GaugeField");
-        hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND, OCC>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
-                           ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
-        KLFT_HIP(hipGetLastError());
-        return KLFT_OK;
-      });
+      if (variant != 0) return ki::set_error(KLFT_E_ARG, "staple-field variant %d is not built", variant);
+      ki::ProfScope prof("staple_field", "stapleField_GaugeField");
+      hipLaunchKernelGGL((k_staple_field<R, NC, COMP, ND, 1>), dim3(nb), dim3(64 * ND), 0, ki::stream(),
+                         ki::links_of<R, NC, COMP>(f), ki::links_of<R, NC, false>(out), f->g);
+      KLFT_HIP(hipGetLastError());
+      return KLFT_OK;
     });
   };
-  rc = launch(ki::pick_variant("staple_field", f, ki::kOccVariants + 1, f->storage == KLFT_STORAGE_F64 ? 3 : 0, launch));
+  // variants: 0 one-term-at-a-time staple, 3 full staple (1, 2: occupancy
+  // targets that never won, removed)
+  rc = launch(ki::pick_variant("staple_field", f, 4, f->storage == KLFT_STORAGE_F64 ? 3 : 0, launch));
   if (rc) return rc;
   return ki::sync_stream();
 }

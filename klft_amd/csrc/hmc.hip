@@ -72,8 +72,8 @@ __global__ __launch_bounds__(256) void k_refresh(R *P, Geom g, uint64_t seed, ui
 // P -= dtau * F with F recomputed from the current links.  Fout (optional):
 // write F in host AoS order instead of updating P.  kin (optional): also
 // reduce 1/2 sum p^2 of the updated momenta.
-template <typename R, int NC, bool COMP, int ND, int OCC, bool LOWREG = true>
-__global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, Geom g,
+template <typename R, int NC, bool COMP, int ND, bool LOWREG>
+__global__ __launch_bounds__(256) void k_force(Links<R, NC, COMP> f, R *P, Geom g,
                                                                      R coef, R dtau, double *Fout,
                                                                      double *partials) {
   f.nd = ND;
@@ -112,49 +112,6 @@ __global__ __launch_bounds__(256, OCC) void k_force(Links<R, NC, COMP> f, R *P, 
   if (partials) block_partial<256>(k, lds, partials);
 }
 
-// The same update with one thread per site for all its directions, one
-// after the other.  The 76 staple-link loads of a site then come from one
-// wave within a short time, so the links its directions share (each pair
-// U_nu(x+mu), U_mu(x+nu) twice, U_nu(x-nu) three times, the site's own links
-// four times: 44 distinct) are re-read from the caches instead of from
-// memory, where the one-thread-per-link kernel spreads them over four waves.
-template <typename R, int NC, bool COMP, int ND>
-__global__ __launch_bounds__(256) void k_force_site(Links<R, NC, COMP> f, R *P, Geom g, R coef, R dtau,
-                                                    double *Fout, double *partials) {
-  f.nd = ND;
-  __shared__ double lds[8];
-  constexpr int NG = Ngen<NC>::value;
-  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
-  int x[ND];
-  long long s;
-  double k = 0.0;
-  if (chunk_site<ND>(g, lb, x, &s)) {
-    const Nbr<ND> nb = make_nbr<ND>(g, x, s);
-#pragma unroll 1
-    for (int mu = 0; mu < ND; ++mu) {
-      const Mat<R, NC> S = staple_lowreg<R, NC, COMP, ND>(f, nb, mu);
-      R F[NG];
-      project(mul(f.load(s, mu), S), coef, F);
-      if (Fout) {
-        const long long site = lex_index<ND>(g, x);
-#pragma unroll
-        for (int a = 0; a < NG; ++a) Fout[(site * ND + mu) * NG + a] = (double)F[a];
-      } else {
-        double n2 = 0.0;
-#pragma unroll
-        for (int a = 0; a < NG; ++a) {
-          R *q = P + ((long long)mu * NG + a) * g.V + s;
-          const R v = *q - dtau * F[a];
-          *q = v;
-          n2 += (double)v * (double)v;
-        }
-        k += 0.5 * n2;
-      }
-    }
-  }
-  if (partials) block_partial<256>(k, lds, partials);
-}
-
 // One leapfrog step with the momentum and link updates fused and the links
 // double-buffered: F from the links in src, P <- P - dtp F, and
 // dst(x,mu) = exp(i dtu P(x,mu)) src(x,mu) (FP32 links reunitarised).  The
@@ -162,8 +119,8 @@ __global__ __launch_bounds__(256) void k_force_site(Links<R, NC, COMP> f, R *P, 
 // still read it; the arithmetic is that of k_force followed by
 // k_update_gauge, so the result is the same.  One thread per link as in
 // k_force: a block holds one 64-site chunk, wave w taking direction w.
-template <typename R, int NC, bool COMP, int ND, bool LOWREG, int OCC = 1>
-__global__ __launch_bounds__(256, OCC) void k_md_step(Links<R, NC, COMP> src, Links<R, NC, COMP> dst, R *P, Geom g,
+template <typename R, int NC, bool COMP, int ND, bool LOWREG>
+__global__ __launch_bounds__(256) void k_md_step(Links<R, NC, COMP> src, Links<R, NC, COMP> dst, R *P, Geom g,
                                                   R coef, R dtp, R dtu) {
   src.nd = ND;
   dst.nd = ND;
@@ -278,42 +235,30 @@ int enqueue_refresh(klft_hmc_t h, uint64_t traj, int slot) {
   });
 }
 
-// variants: 0, 1, 2 one thread per link with >= 1, 2, 4 waves/SIMD; 3 one
-// thread per site (k_force_site); 4 one thread per link with all staple
-// loads in flight at once (full staple)
+// variants: 0 one thread per link, one-term-at-a-time staple; 4 the same with
+// all staple loads in flight at once (full staple).  1-3 (occupancy targets,
+// one thread per site) never won and were removed.
 constexpr int kForceVariants = 5;
 
 int launch_force(klft_hmc_t h, double dtau, double *Fout, int kin_slot, int variant) {
   klft_field_t f = h->f;
+  if (variant != 0 && variant != 4) return ki::set_error(KLFT_E_ARG, "force variant %d is not built", variant);
   // with a kinetic slot, every block writes a partial; k_sum_blocks adds them
   double *parts = kin_slot >= 0 ? f->partials : nullptr;
   return ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
     const auto links = ki::links_of<R, NC, COMP>(f);
     const R coef = (R)(h->p.beta / (double)NC);
-    long long nb = 0;
+    const long long nb = link_blocks<ND>(f->g);
+    if (nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
     {
       ki::ProfScope prof("force", "derivative+update_momentum");
-      if (variant == 3) {
-        nb = ki::blocks_for(chunk_threads<ND>(f->g), 256);
-        if (nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
-        hipLaunchKernelGGL((k_force_site<R, NC, COMP, ND>), dim3((unsigned)nb), dim3(256), 0, ki::stream(), links,
-                           (R *)h->P, f->g, coef, (R)dtau, Fout, parts);
-      } else {
-        nb = link_blocks<ND>(f->g);
-        if (nb > f->max_partials) return ki::set_error(KLFT_E_ARG, "lattice too large for reduction");
-        const int rc = ki::with_occ(variant == 4 ? 0 : variant, [&](auto occ) -> int {
-          constexpr int OCC = decltype(occ)::value;
-          if (variant == 4)
-            hipLaunchKernelGGL((k_force<R, NC, COMP, ND, 1, false>), dim3((unsigned)nb), dim3(64 * ND), 0,
-                               ki::stream(), links, (R *)h->P, f->g, coef, (R)dtau, Fout, parts);
-          else
-            hipLaunchKernelGGL((k_force<R, NC, COMP, ND, OCC>), dim3((unsigned)nb), dim3(64 * ND), 0,
-                               ki::stream(), links, (R *)h->P, f->g, coef, (R)dtau, Fout, parts);
-          return KLFT_OK;
-        });
-        if (rc) return rc;
-      }
+      if (variant == 4)
+        hipLaunchKernelGGL((k_force<R, NC, COMP, ND, false>), dim3((unsigned)nb), dim3(64 * ND), 0, ki::stream(),
+                           links, (R *)h->P, f->g, coef, (R)dtau, Fout, parts);
+      else
+        hipLaunchKernelGGL((k_force<R, NC, COMP, ND, true>), dim3((unsigned)nb), dim3(64 * ND), 0, ki::stream(),
+                           links, (R *)h->P, f->g, coef, (R)dtau, Fout, parts);
       KLFT_HIP(hipGetLastError());
     }
     if (parts) return ki::enqueue_sum_blocks(parts, nb, f->result + kin_slot);
@@ -358,9 +303,8 @@ int enqueue_kinetic(klft_hmc_t h, int slot) {
   });
 }
 
-// variants: 0 one-term-at-a-time staple, 1 full staple (all loads in flight);
-// 2, 3 the same with >= 3 waves per SIMD requested
-constexpr int kMdVariants = 4;
+// variants: 0 one-term-at-a-time staple, 1 full staple (all loads in flight)
+constexpr int kMdVariants = 2;
 
 int launch_md_step(klft_hmc_t h, void *src, void *dst, double dtp, double dtu, int variant) {
   klft_field_t f = h->f;
@@ -374,12 +318,6 @@ int launch_md_step(klft_hmc_t h, void *src, void *dst, double dtp, double dtu, i
     ki::ProfScope prof("md_step", "derivative+update_momentum+update_gauge");
     if (variant == 1)
       hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, false>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,
-                         (R *)h->P, f->g, coef, (R)dtp, (R)dtu);
-    else if (variant == 2)
-      hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, true, 3>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,
-                         (R *)h->P, f->g, coef, (R)dtp, (R)dtu);
-    else if (variant == 3)
-      hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, false, 3>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,
                          (R *)h->P, f->g, coef, (R)dtp, (R)dtu);
     else
       hipLaunchKernelGGL((k_md_step<R, NC, COMP, ND, true>), dim3(nb), dim3(64 * ND), 0, ki::stream(), ls, ld,

@@ -106,16 +106,6 @@ int chosen_variant(const char *kernel, const klft_field_s *f, int nvar, int dflt
 // the variant KLFT_VARIANT forces for `kernel` ("name=v,..."), or -1
 int forced_variant(const char *kernel);
 
-// occupancy variants: v = 0, 1, 2 -> at least 1, 2, 4 waves per SIMD
-constexpr int kOccVariants = 3;
-template <class F>
-int with_occ(int v, F &&fn) {
-  switch (v) {
-    case 1: return fn(ic<2>{});
-    case 2: return fn(ic<4>{});
-    default: return fn(ic<1>{});
-  }
-}
 template <typename T>
 struct tag {
   using type = T;

@@ -149,10 +149,8 @@ struct LinkChain {
 };
 
 // update all directions of the site at position t of sublattice `pass`;
-// returns the number of accepted hits.  HB proposals are drawn before their
-// Markov steps (they do not depend on the field), so their generator rounds
-// overlap; a proposal past nHits is drawn and discarded.
-template <typename R, int NC, bool COMP, int ND, bool DEFECT, bool LOWREG = false, int HB = 1>
+// returns the number of accepted hits
+template <typename R, int NC, bool COMP, int ND, bool DEFECT, bool LOWREG = false>
 __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP> &f, const Geom &g, int pass,
                                                          long long t, int nhits, R beta_nc, R delta,
                                                          uint64_t seed, uint32_t sweep, uint32_t chain,
@@ -171,15 +169,11 @@ __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP
     LinkChain<R, NC> lc;
     lc.start(S, f.load(s, mu));
 #pragma unroll 1
-    for (int hit = 0; hit < nhits; hit += HB) {
-      Mat<R, NC> r[HB];
-      R u[HB];
-#pragma unroll
-      for (int k = 0; k < HB; ++k)
-        propose<R, NC>(seed, site, sweep, (uint32_t)(hit + k), chain, (uint32_t)mu, delta, r[k], u[k]);
-#pragma unroll
-      for (int k = 0; k < HB; ++k)
-        if (k == 0 || hit + k < nhits) lc.step(r[k], u[k], beta_nc, cl, n);
+    for (int hit = 0; hit < nhits; ++hit) {
+      Mat<R, NC> r;
+      R u;
+      propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
+      lc.step(r, u, beta_nc, cl, n);
     }
     lc.apply_pending();
     if (nhits > 0) f.store(s, mu, lc.U);
@@ -310,71 +304,11 @@ __device__ __forceinline__ void term_fold(int m, int t, int defer, const TermLoa
   }
 }
 
-// The same term staged through LDS (FP64 links): the three links' components
-// go global -> LDS with global_load_lds_dwordx4, so they occupy no VGPRs while
-// the hit computes.  `lds` is this wave's region of 3 * NC^2 * 64 complex;
-// lane l's component j of link k sits at lds[(k * NC^2 + j) * 64 + l].
-template <int NC, int ND>
-__device__ __forceinline__ void term_issue_lds(const Links<double, NC, false> &f, const Geom &g, const int *x,
-                                               long long s, int m, int t, int defer, cx<double> *lds) {
-  constexpr int NCOMP = NC * NC;
-  const bool upper = t < ND - 1;
-  const int j = upper ? t : t - (ND - 1);
-  const int nu = j < m ? j : j + 1;
-  long long sa, sb, sc;
-  int da, db, dc;
-  if (upper) {
-    if (nu == defer) return;
-    sa = idx_shift<ND>(g, x, m, 1, -1, 0), da = nu;
-    sb = idx_shift<ND>(g, x, nu, 1, -1, 0), db = m;
-    sc = s, dc = nu;
-  } else {
-    const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
-    sa = idx_shift<ND>(g, x, m, 1, nu, -1), da = nu;
-    sb = s_xmnu, db = m;
-    sc = s_xmnu, dc = nu;
-  }
-  const cx<double> *qa = f.p + f.base(sa, da), *qb = f.p + f.base(sb, db), *qc = f.p + f.base(sc, dc);
-#pragma unroll
-  for (int c = 0; c < NCOMP; ++c) {
-    __builtin_amdgcn_global_load_lds((const void *)(qa + c * 64),
-                                     (__attribute__((address_space(3))) void *)(lds + (0 * NCOMP + c) * 64), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds((const void *)(qb + c * 64),
-                                     (__attribute__((address_space(3))) void *)(lds + (1 * NCOMP + c) * 64), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds((const void *)(qc + c * 64),
-                                     (__attribute__((address_space(3))) void *)(lds + (2 * NCOMP + c) * 64), 16, 0, 0);
-  }
-}
-
-template <int NC>
-__device__ __forceinline__ Mat<double, NC> lds_mat(const cx<double> *lds, int k) {
-  constexpr int NCOMP = NC * NC;
-  const int l = threadIdx.x & 63;
-  Mat<double, NC> u;
-#pragma unroll
-  for (int c = 0; c < NCOMP; ++c) u.m[c / NC][c % NC] = lds[(k * NCOMP + c) * 64 + l];
-  return u;
-}
-
-template <int NC, int ND>
-__device__ __forceinline__ void term_fold_lds(int m, int t, int defer, const cx<double> *lds,
-                                              Mat<double, NC> &acc) {
-  const bool upper = t < ND - 1;
-  const int j = upper ? t : t - (ND - 1);
-  const int nu = j < m ? j : j + 1;
-  if (upper) {
-    if (nu != defer) acc_mul_ad(acc, mul_ad(lds_mat<NC>(lds, 0), lds_mat<NC>(lds, 1)), lds_mat<NC>(lds, 2));
-  } else {
-    acc_mul(acc, mul_dd(lds_mat<NC>(lds, 0), lds_mat<NC>(lds, 1)), lds_mat<NC>(lds, 2));
-  }
-}
-
-// STAGE_LDS (FP64 only): the term's links are staged in the wave's LDS region
-template <typename R, int NC, bool COMP, int ND, bool STAGE_LDS = false>
+template <typename R, int NC, bool COMP, int ND>
 __device__ __forceinline__ unsigned long long metro_site_pipe(const Links<R, NC, COMP> &f, const Geom &g,
                                                               int pass, long long t, int nhits, R beta_nc,
                                                               R delta, uint64_t seed, uint32_t sweep,
-                                                              uint32_t chain, int *visit, cx<R> *lds = nullptr) {
+                                                              uint32_t chain, int *visit) {
   constexpr int NT = 2 * (ND - 1);
   unsigned long long n = 0;
   int x[ND];
@@ -396,18 +330,11 @@ __device__ __forceinline__ unsigned long long metro_site_pipe(const Links<R, NC,
       const bool fold = next && k < NT;
       Mat<R, NC> r;
       R u;
-      if constexpr (STAGE_LDS) {
-        if (fold) term_issue_lds<NC, ND>(f, g, x, s, m, k, mu, lds);
-        propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
-        lc.step(r, u, beta_nc, R(1), n);
-        if (fold) term_fold_lds<NC, ND>(m, k++, mu, lds, acc);
-      } else {
-        TermLoads<R, NC, COMP, ND> L;
-        if (fold) term_load<R, NC, COMP, ND>(f, g, x, s, m, k, mu, L);
-        propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
-        lc.step(r, u, beta_nc, R(1), n);
-        if (fold) term_fold<R, NC, COMP, ND>(m, k++, mu, L, acc);
-      }
+      TermLoads<R, NC, COMP, ND> L;
+      if (fold) term_load<R, NC, COMP, ND>(f, g, x, s, m, k, mu, L);
+      propose<R, NC>(seed, site, sweep, (uint32_t)hit, chain, (uint32_t)mu, delta, r, u);
+      lc.step(r, u, beta_nc, R(1), n);
+      if (fold) term_fold<R, NC, COMP, ND>(m, k++, mu, L, acc);
     }
     if (next) {
 #pragma unroll 1

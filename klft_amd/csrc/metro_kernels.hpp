@@ -18,10 +18,9 @@ using namespace klft_internal;
 
 constexpr int kMetroBlock = 256;
 
-// LOWREG takes the staple one term at a time; HB proposals are drawn ahead of
-// their Markov steps; OCC asks the register allocator for at least OCC waves
-// per SIMD.
-template <typename R, int NC, bool COMP, int ND, bool LOWREG, int HB, int OCC>
+// LOWREG takes the staple one term at a time; OCC asks the register allocator
+// for at least OCC waves per SIMD.
+template <typename R, int NC, bool COMP, int ND, bool LOWREG, int OCC>
 __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_pass(
     Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
@@ -31,7 +30,7 @@ __global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_pass(
   const long long t = (long long)lb * kMetroBlock + threadIdx.x;
   unsigned long long n = 0;
   if (t < g.Vh)
-    n = metro_site<R, NC, COMP, ND, false, LOWREG, HB>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep,
+    n = metro_site<R, NC, COMP, ND, false, LOWREG>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep,
                                                        chain, Defect{}, R(1), visit);
   n = wave_sum_u64(n);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
@@ -79,33 +78,6 @@ __global__ __launch_bounds__(kMetroBlock) void k_metropolis_own(
   unsigned long long n = 0;
   if (t < g.Vh)
     n = metro_site_own<R, NC, COMP, ND>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain, visit);
-  n = wave_sum_u64(n);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long b = 0;
-#pragma unroll
-    for (int i = 0; i < kMetroBlock / 64; ++i) b += red[i];
-    if (b) atomicAdd(nacc, b);
-  }
-}
-
-// the same with the next staple's links staged through LDS (FP64 links)
-template <int NC, int ND>
-__global__ __launch_bounds__(kMetroBlock) void k_metropolis_pipe_lds(
-    Links<double, NC, false> f, Geom g, int pass, int nhits, double beta_nc, double delta, uint64_t seed,
-    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
-  f.nd = ND;
-  __shared__ unsigned long long red[kMetroBlock / 64];
-  __shared__ cx<double> stage[kMetroBlock / 64][3 * NC * NC * 64];
-  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
-  const long long t = (long long)lb * kMetroBlock + threadIdx.x;
-  unsigned long long n = 0;
-  // each lane's LDS transfers land in, and are read from, its own slots of
-  // the wave's region, so lanes past Vh (which issue none) disturb nothing
-  if (t < g.Vh)
-    n = metro_site_pipe<double, NC, false, ND, true>(f, g, pass, t, nhits, beta_nc, delta, seed, sweep, chain,
-                                                     visit, stage[threadIdx.x >> 6]);
   n = wave_sum_u64(n);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
   __syncthreads();
@@ -443,151 +415,14 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
   ws_count(n, red, nacc);
 }
 
-// ---------------------------------------------------------------------------
-// Group variants: four (quad) or two (pair) lanes per site.  The proposals
-// (and their accept draws) do not depend on the field, so for hits
-// h0..h0+LANES-1 lane q generates the proposal of hit h0+q from its own keyed
-// stream; the group then steps the Markov chain through those hits in order,
-// each proposal broadcast from its lane with a DPP quad permute, every lane
-// computing the same decision and the same updated link.  The 2(nd-1) staple
-// terms are split over the lanes (lane q: terms q, q+LANES, ...) and summed
-// with commutative DPP exchanges, so all lanes hold the bitwise-identical
-// staple.  LANES times the waves of the one-lane kernel (a 32^4 pass has only
-// one wave per SIMD otherwise) and 1/LANES of its generator work per lane.
-// ---------------------------------------------------------------------------
-template <int CTRL>
-__device__ __forceinline__ float dpp(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <int CTRL, typename R, int NC>
-__device__ __forceinline__ Mat<R, NC> dpp_mat(const Mat<R, NC> &a) {
-  Mat<R, NC> b;
-#pragma unroll
-  for (int i = 0; i < NC; ++i)
-#pragma unroll
-    for (int j = 0; j < NC; ++j) b.m[i][j] = mk(dpp<CTRL>(a.m[i][j].re), dpp<CTRL>(a.m[i][j].im));
-  return b;
-}
-// broadcast of lane J of each group of LANES lanes: quad_perm [J, J, J, J]
-// for quads, [J, J, 2+J, 2+J] for pairs
-template <int LANES, int J>
-struct BcastCtrl {
-  static constexpr int value = LANES == 4 ? J * 0x55 : (J == 0 ? 0xA0 : 0xF5);
-};
-template <int LANES, int J, typename R, int NC>
-__device__ __forceinline__ Mat<R, NC> bcast(const Mat<R, NC> &a) {
-  return dpp_mat<BcastCtrl<LANES, J>::value>(a);
-}
-template <int LANES, int J, typename R>
-__device__ __forceinline__ R bcast_s(R v) {
-  return dpp<BcastCtrl<LANES, J>::value>(v);
-}
-
-// partial staple of lane q: terms q, q + LANES, ... of [up(nu_0..), low(nu_0..)]
-template <typename R, int NC, bool COMP, int ND, int LANES>
-__device__ __forceinline__ Mat<R, NC> staple_part(const Links<R, NC, COMP> &f, const Geom &g,
-                                                  const int *x, int mu, int q) {
-  constexpr int NT = 2 * (ND - 1);
-  Mat<R, NC> t = zero<R, NC>();
-#pragma unroll 1
-  for (int k = q; k < NT; k += LANES) {
-    const bool upper = k < ND - 1;
-    const int j = upper ? k : k - (ND - 1);
-    const int nu = j < mu ? j : j + 1;  // j-th direction other than mu
-    if (upper) {
-      const Mat<R, NC> ab = mul_ad(f.load(idx_shift<ND>(g, x, mu, 1, -1, 0), nu),
-                                   f.load(idx_shift<ND>(g, x, nu, 1, -1, 0), mu));
-      acc_mul_ad(t, ab, f.load(dev_index<ND>(g, x), nu));
-    } else {
-      const long long s_xmnu = idx_shift<ND>(g, x, nu, -1, -1, 0);
-      const Mat<R, NC> ab = mul_dd(f.load(idx_shift<ND>(g, x, mu, 1, nu, -1), nu), f.load(s_xmnu, mu));
-      acc_mul(t, ab, f.load(s_xmnu, nu));
-    }
-  }
-  return t;
-}
-
-// sum over the group's lanes; each exchange is commutative, so every lane
-// of the group holds the bitwise-identical sum
-template <int LANES, typename R, int NC>
-__device__ __forceinline__ Mat<R, NC> group_sum(const Mat<R, NC> &a) {
-  Mat<R, NC> t = a;
-  add_to(t, dpp_mat<0xB1>(t));                           // quad_perm [1,0,3,2]
-  if constexpr (LANES == 4) add_to(t, dpp_mat<0x4E>(t));  // quad_perm [2,3,0,1]
-  return t;
-}
-
-// LANES = 4 (quad) or 2 (pair) lanes per site
-template <typename R, int NC, bool COMP, int ND, int OCC, int LANES>
-__global__ __launch_bounds__(kMetroBlock, OCC) void k_metropolis_group(
-    Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
-    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
-  f.nd = ND;
-  static_assert(LANES == 2 || LANES == 4, "two or four lanes per site");
-  __shared__ unsigned long long red[kMetroBlock / 64];
-  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
-  const long long t = ((long long)lb * kMetroBlock + threadIdx.x) / LANES;
-  const int q = threadIdx.x & (LANES - 1);
-  // a whole group is either in range or not, so the DPP exchanges never read
-  // an inactive lane
-  const bool active = t < g.Vh;
-  unsigned long long n = 0;
-  if (active) {
-    int x[ND];
-    sub_coords<ND>(g, pass, t, x);
-    const long long s = (long long)pass * g.Vh + t;
-    const uint32_t site = (uint32_t)lex_index<ND>(g, x);
-    if (visit && q == 0) visit[site] = pass;
-#pragma unroll 1
-    for (int mu = 0; mu < ND; ++mu) {
-      LinkChain<R, NC> lc;
-      lc.start(group_sum<LANES>(staple_part<R, NC, COMP, ND, LANES>(f, g, x, mu, q)), f.load(s, mu));
-#pragma unroll 1
-      for (int h0 = 0; h0 < nhits; h0 += LANES) {
-        Mat<R, NC> r = identity<R, NC>();
-        R u = R(1);
-        if (h0 + q < nhits)
-          propose<R, NC>(seed, site, sweep, (uint32_t)(h0 + q), chain, (uint32_t)mu, delta, r, u);
-        lc.step(bcast<LANES, 0>(r), bcast_s<LANES, 0>(u), beta_nc, R(1), n);
-        if (h0 + 1 < nhits) lc.step(bcast<LANES, 1>(r), bcast_s<LANES, 1>(u), beta_nc, R(1), n);
-        if constexpr (LANES == 4) {
-          if (h0 + 2 < nhits) lc.step(bcast<LANES, 2>(r), bcast_s<LANES, 2>(u), beta_nc, R(1), n);
-          if (h0 + 3 < nhits) lc.step(bcast<LANES, 3>(r), bcast_s<LANES, 3>(u), beta_nc, R(1), n);
-        }
-      }
-      lc.apply_pending();
-      if (nhits > 0 && q == 0) f.store(s, mu, lc.U);
-    }
-    if (q != 0) n = 0;
-  }
-  n = wave_sum_u64(n);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long b = 0;
-#pragma unroll
-    for (int i = 0; i < kMetroBlock / 64; ++i) b += red[i];
-    if (b) atomicAdd(nacc, b);
-  }
-}
-
-
-// launch passes [p0, p1) of one sweep on the link array `links` with
-// variant 0 (one lane per site), 1..3 (quad per site, occupancy 1, 2, 4),
-// 4 (one lane, low-register staple, occupancy 2), 5 (one lane, proposals
-// drawn in pairs), 6 (one lane, low-register staple, pairs), 7 (one lane,
-// next staple loaded during the hits), 8 (as 7, FP64 links staged in LDS)
-// 9, 10 (pair per site, occupancy 2, 4), 11 (one lane, own links kept
-// in registers), 12 (site waves fed by staple waves) or 13 (as 12, each
-// direction's upper and lower staple terms back to back); 14, 15: 12, 13
-// with the symmetric SU(3) proposal
+// Sweep variants (numbers kept stable for tuning caches): 0 one lane per
+// site; 4 one lane, low-register staple, >= 2 waves per SIMD; 7 one lane,
+// next direction's staple loaded during the hits; 11 one lane, the site's own
+// links kept in registers; 12 site waves fed by staple waves; 13 the same with
+// each direction's upper and lower staple terms back to back; 14, 15: 12, 13
+// with the symmetric SU(3) proposal.  1-3, 5, 6, 8-10 (four or two lanes per
+// site, proposals drawn in pairs, LDS-staged staple terms) never won at
+// 16^4 - 48^4 and were removed.
 template <typename RR, bool CC>
 int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1,
                         const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
@@ -599,9 +434,11 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
     l.V = f->g.V;
     l.nd = f->nd;
     const R beta_nc = (R)(p->beta / (double)NC);
-    const bool quad = variant >= 1 && variant <= 3;
-    const bool pair = variant == 9 || variant == 10;
-    const unsigned nb = blocks_for(f->g.Vh * (quad ? 4 : (pair ? 2 : 1)), kMetroBlock);
+    const unsigned nb = blocks_for(f->g.Vh, kMetroBlock);
+    switch (variant) {
+      case 0: case 4: case 7: case 11: case 12: case 13: case 14: case 15: break;
+      default: return set_error(KLFT_E_ARG, "sweep variant %d is not built", variant);
+    }
     for (int pass = p0; pass < p1; ++pass) {
       char label[48];
       snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pass);  // Metropolis.hpp:161
@@ -628,40 +465,16 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
         hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, true>), dim3(blocks_for(f->g.Vh, kWsSites)),
                            dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
                            (uint32_t)sweep, chain, counter, visit);
-      else if (quad)
-        with_occ(variant - 1, [&](auto occ) -> int {
-          constexpr int OCC = decltype(occ)::value;
-          hipLaunchKernelGGL((k_metropolis_group<R, NC, COMP, ND, OCC, 4>), dim3(nb), dim3(kMetroBlock), 0,
-                             stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
-                             (uint32_t)sweep, chain, counter, visit);
-          return KLFT_OK;
-        });
-      else if (pair)
-        with_occ(variant - 8, [&](auto occ) -> int {  // 9: >= 2, 10: >= 4 waves/SIMD
-          constexpr int OCC = decltype(occ)::value;
-          hipLaunchKernelGGL((k_metropolis_group<R, NC, COMP, ND, OCC, 2>), dim3(nb), dim3(kMetroBlock), 0,
-                             stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
-                             (uint32_t)sweep, chain, counter, visit);
-          return KLFT_OK;
-        });
       else {
         auto one_lane = [&](auto kern) {
           hipLaunchKernelGGL(kern, dim3(nb), dim3(kMetroBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc,
                              (R)p->delta, p->seed, (uint32_t)sweep, chain, counter, visit);
         };
         switch (variant) {
-          case 4: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 1, 2>); break;
-          case 5: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 2, 1>); break;
-          case 6: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2, 1>); break;
+          case 4: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2>); break;
           case 7: one_lane(k_metropolis_pipe<R, NC, COMP, ND>); break;
           case 11: one_lane(k_metropolis_own<R, NC, COMP, ND>); break;
-          case 8:
-            if constexpr (std::is_same<R, double>::value && !COMP)
-              one_lane(k_metropolis_pipe_lds<NC, ND>);
-            else
-              one_lane(k_metropolis_pipe<R, NC, COMP, ND>);
-            break;
-          default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1, 1>); break;
+          default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1>); break;
         }
       }
     }

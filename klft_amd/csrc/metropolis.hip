@@ -64,15 +64,13 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
 // scratch copy of the links, so the live field is never updated by it
 int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool timed) {
   void *scratch = nullptr;
-  // variants: 0 one lane per site; 1, 2, 3 quad per site with >= 1, 2, 4
-  // waves/SIMD; 4 one lane with the low-register staple at >= 2 waves/SIMD;
-  // 5, 6 one lane drawing proposals in pairs, with the full / low-register
-  // staple; 7 one lane building the next direction's staple during the hits;
-  // 8 the same with FP64 links staged through LDS (7 for FP32 storage);
-  // 9, 10 a pair of lanes per site with >= 2, 4 waves/SIMD; 11 one lane
-  // with the site's own links kept in registers; 12 site waves fed the next
-  // staple by staple waves; 13 the same with each direction's upper and
-  // lower staple terms back to back.
+  // variants (metro_kernels.hpp): 0 one lane per site; 4 one lane with the
+  // low-register staple at >= 2 waves/SIMD; 7 one lane building the next
+  // direction's staple during the hits; 11 one lane with the site's own links
+  // kept in registers; 12 site waves fed the next staple by staple waves; 13
+  // the same with each direction's upper and lower staple terms back to back.
+  // The other numbers below kMetroVariants are not built (the tuner skips
+  // them).
   // Untuned defaults: site waves fed by staple waves, the fastest at 16^4,
   // 24^4 and 32^4 for every group (nHits = 10), with the interleaved term
   // order for FP64 links and the upper-terms-first order for FP32 links
@@ -80,6 +78,9 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool tim
   const int dflt = f->storage == KLFT_STORAGE_F64 ? 13 : 12;
   // the symmetric SU(3) proposal runs in the site-wave kernels only
   if (f->proposal == KLFT_PROPOSAL_SYMMETRIC && f->nc == 3) return dflt + 2;
+  // KLFT_VARIANT="metropolis=v" forces a variant for every hit count
+  const int fv = forced_variant("metropolis");
+  if (fv >= 0 && fv < kMetroVariants) return fv;
   // the tuning key carries nHits: the best variant depends on the hit count
   const std::string kname = "metropolis_nhits_" + std::to_string(p->nhits);
   if (!timed) return chosen_variant(kname.c_str(), f, kMetroVariants, dflt);
