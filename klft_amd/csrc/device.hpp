@@ -5,6 +5,7 @@
 //
 // Link layout in HBM (one allocation per field), blocks of 64 sites:
 //   element (s, mu, c) at ((s / 64) * nd + mu) * NCOMP * 64 + c * 64 + s % 64
+//   (FP32: components paired per lane, see Links below)
 //   s   = sub * Vh + r  (device index)
 //   sub = sum_d (x_d & 1) << (nd - 1 - d)   -- the sublattice (pass) index of
 //         the reference's oddeven_array (IndexHelper.hpp:84-93)
@@ -547,28 +548,59 @@ __device__ __forceinline__ Nbr<ND> make_nbr(const Geom& g, const int* x, long lo
 // ---------------------------------------------------------------------------
 // Array-of-structs-of-arrays: sites are grouped in blocks of 64 consecutive
 // device indices, and a block stores all directions and colour components of
-// its 64 sites contiguously,
-//   element (s, mu, c) at ((s / 64) * nd + mu) * NCOMP * 64 + c * 64 + s % 64,
-// so a wave's load of one link for 64 sites is one contiguous run of
-// NCOMP * 64 complex numbers (9 KiB for FP64 SU(3)) instead of NCOMP runs in
-// arrays a field-size apart.
+// its 64 sites contiguously, one run of NCOMP * 64 complex numbers per
+// (block, mu) at ((s / 64) * nd + mu) * NCOMP * 64, so a wave's load of one
+// link for 64 sites reads one contiguous run (9 KiB for FP64 SU(3)) instead of
+// NCOMP runs in arrays a field-size apart.  Inside a run:
+//   FP64: component c of lane l at c * 64 + l (one 16-byte complex per lane
+//         and load instruction);
+//   FP32: components 2k and 2k + 1 of lane l side by side at k * 128 + 2 l
+//         (+1), and for odd NCOMP the last one at (NCOMP - 1) * 64 + l, so a
+//         lane loads 16 bytes per instruction there too: an FP32 link takes 5
+//         load instructions instead of 9 (two-row SU(3): 3 instead of 6).
+//         The vector memory unit's work scales with the number of
+//         instructions more than with their width.
 template <typename R, int NC, bool COMP>
 struct Links {
   static constexpr bool C = COMP && NC > 1;
   static constexpr int NCOMP = C ? (NC - 1) * NC : NC * NC;
+  static constexpr bool PAIRED = sizeof(R) == 4;
+  static constexpr int NPAIR = PAIRED ? NCOMP / 2 : 0;
   cx<R>* __restrict__ p;
   long long V;
   int nd;  // kernels templated on the direction count set nd = ND on entry,
            // which makes the index arithmetic below compile-time
 
-  __device__ __forceinline__ long long base(long long s, int mu) const {
-    return (((s >> 6) * nd + mu) * NCOMP) * 64 + (s & 63);
+  // start of the (block, mu) run holding site s
+  __device__ __forceinline__ long long run(long long s, int mu) const {
+    return (((s >> 6) * nd + mu) * NCOMP) * 64;
+  }
+  // element offset of component c of link (s, mu)
+  __device__ __forceinline__ long long comp_index(long long s, int mu, int c) const {
+    const long long l = s & 63;
+    if constexpr (PAIRED) {
+      if (c < 2 * NPAIR) return run(s, mu) + (c >> 1) * 128 + 2 * l + (c & 1);
+      return run(s, mu) + (NCOMP - 1) * 64 + l;
+    } else {
+      return run(s, mu) + c * 64 + l;
+    }
   }
   __device__ __forceinline__ Mat<R, NC> load(long long s, int mu) const {
     Mat<R, NC> u;
-    const cx<R>* q = p + base(s, mu);
+    const cx<R>* q = p + run(s, mu);
+    const int l = (int)(s & 63);
+    if constexpr (PAIRED) {
 #pragma unroll
-    for (int c = 0; c < NCOMP; ++c) u.m[c / NC][c % NC] = q[c * 64];
+      for (int k = 0; k < NPAIR; ++k) {
+        const float4 v = reinterpret_cast<const float4*>(q + k * 128)[l];
+        u.m[(2 * k) / NC][(2 * k) % NC] = mk<R>(v.x, v.y);
+        u.m[(2 * k + 1) / NC][(2 * k + 1) % NC] = mk<R>(v.z, v.w);
+      }
+      if constexpr (NCOMP & 1) u.m[(NCOMP - 1) / NC][(NCOMP - 1) % NC] = q[(NCOMP - 1) * 64 + l];
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) u.m[c / NC][c % NC] = q[c * 64 + l];
+    }
     if constexpr (C) {
       if constexpr (NC == 3) complete_row2(u);
       if constexpr (NC == 2) complete_row1(u);
@@ -576,9 +608,19 @@ struct Links {
     return u;
   }
   __device__ __forceinline__ void store(long long s, int mu, const Mat<R, NC>& u) const {
-    cx<R>* q = p + base(s, mu);
+    cx<R>* q = p + run(s, mu);
+    const int l = (int)(s & 63);
+    if constexpr (PAIRED) {
 #pragma unroll
-    for (int c = 0; c < NCOMP; ++c) q[c * 64] = u.m[c / NC][c % NC];
+      for (int k = 0; k < NPAIR; ++k) {
+        const cx<R> a = u.m[(2 * k) / NC][(2 * k) % NC], b = u.m[(2 * k + 1) / NC][(2 * k + 1) % NC];
+        reinterpret_cast<float4*>(q + k * 128)[l] = make_float4(a.re, a.im, b.re, b.im);
+      }
+      if constexpr (NCOMP & 1) q[(NCOMP - 1) * 64 + l] = u.m[(NCOMP - 1) / NC][(NCOMP - 1) % NC];
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) q[c * 64 + l] = u.m[c / NC][c % NC];
+    }
   }
 };
 

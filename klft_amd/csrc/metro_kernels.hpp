@@ -131,10 +131,9 @@ __device__ __forceinline__ Mat<R, NC> lds_get(const cx<R> *buf) {
 template <typename R, int NC, bool COMP>
 __device__ __forceinline__ void store_from_lds(const Links<R, NC, COMP> &f, long long s, int mu,
                                                const cx<R> *buf) {
-  cx<R> *q = f.p + f.base(s, mu);
   const int l = threadIdx.x & 63;
 #pragma unroll
-  for (int c = 0; c < Links<R, NC, COMP>::NCOMP; ++c) q[c * 64] = buf[c * 64 + l];
+  for (int c = 0; c < Links<R, NC, COMP>::NCOMP; ++c) f.p[f.comp_index(s, mu, c)] = buf[c * 64 + l];
 }
 
 // LinkChain with U in the site wave's LDS buffer `bu`: U is read only when
