@@ -71,11 +71,15 @@ def metropolis_leg(klft, st, a, dims):
     p = klft.MetropolisParams(nHits=a.nhits, beta=a.beta, delta=a.delta, seed=4242)
     if a.warmup:
         klft.metropolis_run(g, p, 0, a.warmup, rates=False)
-    klft.profile(True)
+    # the timed sweeps run as a caller runs them, without per-launch events
     t = sync_time(lambda: klft.metropolis_run(g, p, a.warmup, a.steps, rates=False))
+    # pass durations: HIP events around every pass launch, in sweeps of their own
+    nprof = max(2, min(a.steps, 5))
+    klft.profile(True)
+    klft.metropolis_run(g, p, a.warmup + a.steps, nprof, rates=False)
     ms, n = klft.profile_read("metropolis_pass")
     klft.profile(False)
-    acc = klft.sweep_Metropolis(g, p, a.warmup + a.steps)
+    acc = klft.sweep_Metropolis(g, p, a.warmup + a.steps + nprof)
     plaq = klft.GaugePlaquette(g)
     lu = 4 * V * a.steps / t
     pass_us = 1e3 * ms / max(n, 1)
@@ -94,9 +98,12 @@ def hmc_leg(klft, st, a, dims):
     h = klft.HMC(g, beta=a.beta, tau=1.0, nsteps=20, seed=99)
     for i in range(a.hmc_warmup):
         h.trajectory(i)
-    klft.profile(True)
     res = []
     t = sync_time(lambda: res.extend(h.trajectory(a.hmc_warmup + i) for i in range(a.hmc_traj)))
+    # kernel durations from two further trajectories with HIP events on
+    klft.profile(True)
+    for i in range(2):
+        h.trajectory(a.hmc_warmup + a.hmc_traj + i)
     mms, mn = klft.profile_read("md_step")
     fms, fn = klft.profile_read("force")
     klft.profile(False)

@@ -168,6 +168,7 @@ __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP
     if constexpr (DEFECT) cl = on_defect<ND>(g, d, x, mu) ? c : R(1);
     LinkChain<R, NC> lc;
     lc.start(S, f.load(s, mu));
+    const unsigned long long n0 = n;
 #pragma unroll 1
     for (int hit = 0; hit < nhits; ++hit) {
       Mat<R, NC> r;
@@ -176,7 +177,8 @@ __device__ __forceinline__ unsigned long long metro_site(const Links<R, NC, COMP
       lc.step(r, u, beta_nc, cl, n);
     }
     lc.apply_pending();
-    if (nhits > 0) f.store(s, mu, lc.U);
+    // only a link with an accepted hit has changed
+    if (n != n0) f.store(s, mu, lc.U);
   }
   return n;
 }
@@ -325,6 +327,7 @@ __device__ __forceinline__ unsigned long long metro_site_pipe(const Links<R, NC,
     const bool next = m < ND;
     Mat<R, NC> acc = zero<R, NC>();
     int k = 0;
+    const unsigned long long n0 = n;
 #pragma unroll 1
     for (int hit = 0; hit < nhits; ++hit) {
       const bool fold = next && k < NT;
@@ -345,7 +348,7 @@ __device__ __forceinline__ unsigned long long metro_site_pipe(const Links<R, NC,
       }
     }
     lc.apply_pending();
-    if (nhits > 0) f.store(s, mu, lc.U);
+    if (n != n0) f.store(s, mu, lc.U);
     if (next) {
       // the upper term through U_mu(x), now final
       const Mat<R, NC> ab = mul_ad(f.load(idx_shift<ND>(g, x, m, 1, -1, 0), mu),

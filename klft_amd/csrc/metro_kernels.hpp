@@ -352,6 +352,7 @@ __device__ __forceinline__ unsigned long long ws_pass(const Links<R, NC, COMP> &
       __syncthreads();
       if (active) {
         const R cl = cl_of(x, mu);
+        const unsigned long long n0 = n;
 #pragma unroll 1
         for (int hit = 0; hit < nhits; ++hit) {
           Mat<R, NC> r;
@@ -360,7 +361,10 @@ __device__ __forceinline__ unsigned long long ws_pass(const Links<R, NC, COMP> &
           lc.step(r, u, beta_nc, cl, n);
         }
         lc.apply_pending();
-        if (nhits > 0) store_from_lds(f, s, mu, lc.bu);
+        // a link none of whose hits was accepted is unchanged in memory, so
+        // only the lanes that accepted write theirs back (at beta = 6 about
+        // one link in ten changes per pass)
+        if (n != n0) store_from_lds(f, s, mu, lc.bu);
       }
       __syncthreads();
     }
