@@ -73,9 +73,19 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool tim
   // them).
   // Untuned defaults: site waves fed by staple waves, the fastest at 16^4,
   // 24^4 and 32^4 for every group (nHits = 10), with the interleaved term
-  // order for FP64 links and the upper-terms-first order for FP32 links
-  // (the interleaved order loses at 48^4 in FP32).
-  const int dflt = f->storage == KLFT_STORAGE_F64 ? 13 : 12;
+  // order for FP64 links.  For FP32 links the interleaved order's six loads
+  // in flight (196 VGPRs) leave room for one block per CU, the
+  // upper-terms-first order (128 VGPRs) for two: the interleaved order wins
+  // while a pass fills at most one block per CU (32^4: 72 against 76 us) and
+  // loses once blocks queue for a CU (48^4: 398 against 308 us).
+  static const int ncu = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 256;
+    return n > 0 ? n : 256;
+  }();
+  const bool one_round = (f->g.Vh + 255) / 256 <= (long long)ncu;
+  const int dflt = (f->storage == KLFT_STORAGE_F64 || one_round) ? 13 : 12;
   // the symmetric SU(3) proposal runs in the site-wave kernels only
   if (f->proposal == KLFT_PROPOSAL_SYMMETRIC && f->nc == 3) return dflt + 2;
   // KLFT_VARIANT="metropolis=v" forces a variant for every hit count

@@ -100,6 +100,7 @@ typedef struct {
   uint32_t block;
   uint32_t buf[4];
   int pos;
+  int narrow; /* uniforms of 24 bits from one word (the FP32 draws) */
 } stream;
 
 static void stream_init(stream *s, uint64_t seed, uint32_t tag, uint32_t c0,
@@ -112,6 +113,7 @@ static void stream_init(stream *s, uint64_t seed, uint32_t tag, uint32_t c0,
   s->hi = (tag << 28) | (chain << 18) | (mu << 16);
   s->block = 0;
   s->pos = 4;
+  s->narrow = 0;
 }
 
 static uint32_t next_word(stream *s) {
@@ -125,6 +127,8 @@ static uint32_t next_word(stream *s) {
 }
 
 static double next_u01(stream *s) {
+  /* FP32 links draw one word per uniform, 24 bits: exact in float and double */
+  if (s->narrow) return (double)(next_word(s) >> 8) * (1.0 / 16777216.0);
   uint32_t a = next_word(s) >> 5;
   uint32_t b = next_word(s) >> 6;
   return ((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0);
@@ -749,7 +753,7 @@ static void rand_sym3(stream *s, double delta, sun *r) {
 
 static double metropolis_impl(const lat *l, double *U, const double *C, int nhits, double beta,
                               double delta, uint64_t seed, uint32_t sweep, uint32_t chain,
-                              long *naccepted, int *visit, int sym) {
+                              long *naccepted, int *visit, int sym, int narrow) {
   int nd = l->nd, nc = l->nc;
   long H[4] = {1, 1, 1, 1}, Vh = 1;
   for (int d = 0; d < nd; ++d) {
@@ -780,9 +784,11 @@ static double metropolis_impl(const lat *l, double *U, const double *C, int nhit
           sun rr;
           if (sym && nc == 3) {
             stream_init(&st, seed, 9, (uint32_t)site, sweep, (uint32_t)hit, chain, (uint32_t)mu);
+            st.narrow = narrow;
             rand_sym3(&st, delta, &rr);
           } else {
             stream_init(&st, seed, 1, (uint32_t)site, sweep, (uint32_t)hit, chain, (uint32_t)mu);
+            st.narrow = narrow;
             rand_sun(nc, &st, delta, &rr);
           }
           sun Uold = link(l, U, site, mu);
@@ -829,13 +835,22 @@ double orc_metropolis_sweep(int nd, int nc, const int *dims, double *U, int nhit
                             double delta, uint64_t seed, uint32_t sweep, long *naccepted,
                             int *visit) {
   lat l = mklat(nd, nc, dims);
-  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, visit, 0);
+  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, visit, 0, 0);
 }
 
 double orc_metropolis_sweep_sym(int nd, int nc, const int *dims, double *U, int nhits, double beta,
                                 double delta, uint64_t seed, uint32_t sweep, long *naccepted) {
   lat l = mklat(nd, nc, dims);
-  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, NULL, 1);
+  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, NULL, 1, 0);
+}
+
+/* The same sweep with the FP32 links' draws: every uniform is 24 bits of one
+ * stream word (the library's FP32 draw), so the proposals and accept draws
+ * are those of the FP32 kernels, with FP64 arithmetic throughout. */
+double orc_metropolis_sweep_narrow(int nd, int nc, const int *dims, double *U, int nhits, double beta,
+                                   double delta, uint64_t seed, uint32_t sweep, int sym, long *naccepted) {
+  lat l = mklat(nd, nc, dims);
+  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, NULL, sym, 1);
 }
 
 int orc_rand_sym3(double delta, uint64_t seed, uint32_t sweep, uint32_t site, uint32_t hit,
@@ -1042,5 +1057,5 @@ double orc_metropolis_sweep_defect(int nd, int nc, const int *dims, double *U, c
                                    int nhits, double beta, double delta, uint64_t seed,
                                    uint32_t sweep, uint32_t chain, long *naccepted) {
   lat l = mklat(nd, nc, dims);
-  return metropolis_impl(&l, U, C, nhits, beta, delta, seed, sweep, chain, naccepted, NULL, 0);
+  return metropolis_impl(&l, U, C, nhits, beta, delta, seed, sweep, chain, naccepted, NULL, 0, 0);
 }

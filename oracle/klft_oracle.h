@@ -79,6 +79,10 @@ double orc_metropolis_sweep(int nd, int nc, const int *dims, double *U,
  * r = exp(i delta sum_a h_a lambda_a), h_a = drand(-1, 1) */
 double orc_metropolis_sweep_sym(int nd, int nc, const int *dims, double *U, int nhits, double beta,
                                 double delta, uint64_t seed, uint32_t sweep, long *naccepted);
+/* orc_metropolis_sweep with the FP32 links' draws (24-bit uniforms, one
+ * stream word each); sym != 0 draws the symmetric SU(3) proposal */
+double orc_metropolis_sweep_narrow(int nd, int nc, const int *dims, double *U, int nhits, double beta,
+                                   double delta, uint64_t seed, uint32_t sweep, int sym, long *naccepted);
 int orc_rand_sym3(double delta, uint64_t seed, uint32_t sweep, uint32_t site, uint32_t hit,
                   uint32_t mu, double *out);
 

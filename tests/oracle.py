@@ -173,6 +173,21 @@ def metropolis_sweep_sym(U, nd, nc, dims, nhits, beta, delta, seed, sweep):
     return acc, n.value
 
 
+L.orc_metropolis_sweep_narrow.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_double,
+                                          C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_long)]
+L.orc_metropolis_sweep_narrow.restype = C.c_double
+
+
+def metropolis_sweep_narrow(U, nd, nc, dims, nhits, beta, delta, seed, sweep, sym=False):
+    """Metropolis sweep with the FP32 links' draws (24-bit uniforms, one stream
+    word each) in FP64 arithmetic, in place on U; returns (acc, naccepted)."""
+    assert U.flags["C_CONTIGUOUS"] and U.dtype == np.complex128
+    n = C.c_long()
+    acc = L.orc_metropolis_sweep_narrow(nd, nc, _i(dims), _d(U), nhits, beta, delta, seed, sweep, int(sym),
+                                        C.byref(n))
+    return acc, n.value
+
+
 def rand_sym3(delta, seed, sweep, site, hit, mu):
     out = np.empty((3, 3), dtype=np.complex128)
     L.orc_rand_sym3(delta, seed, sweep, site, hit, mu, _d(out))

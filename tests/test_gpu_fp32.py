@@ -152,3 +152,44 @@ def test_fp32_hmc_trajectory(nc, beta, storage):
             return
     report(f"HMC dH nc={nc} {storage} (relative to 1 + |dH|)", worst, 2e-3)
     report(f"HMC links after 2 trajectories nc={nc} {storage}", float(np.abs(g.download() - Uo).max()), 1e-4)
+
+
+# FP32 Metropolis decision by decision.  The FP32 kernels draw 24-bit
+# uniforms, one stream word each; orc.metropolis_sweep_narrow runs the same
+# draws through the reference's algorithm in FP64 (Metropolis.hpp:72-113,
+# SUN.hpp:163-251), so the two chains take the same accept decisions except
+# where FP32 rounding of dS moves a decision across its threshold, which
+# changes that link and, through its staples, its neighbours' later hits.
+# Bounds: the accept counts of each sweep within 0.5 % (at least 2), links
+# that agree to 1e-4 at least 97 % of all links, and on those the largest
+# difference below 5e-6 (FP32 rounding and re-unitarisation; measured: identical
+# counts, every link within 7e-7).
+MET_CASES = [(4, 3, (6, 6, 6, 6), 5.5, 0.2, False), (4, 3, (6, 6, 6, 6), 5.5, 0.2, True),
+             (4, 2, (8, 8, 8, 8), 2.3, 0.3, False), (4, 1, (8, 8, 8, 8), 2.0, 0.5, False),
+             (3, 3, (8, 6, 8), 5.0, 0.2, False)]
+
+
+@pytest.mark.parametrize("storage", STORAGES)
+@pytest.mark.parametrize("nd,nc,dims,beta,delta,sym", MET_CASES)
+def test_fp32_metropolis_tracks_oracle_decisions(storage, nd, nc, dims, beta, delta, sym):
+    U = orc.random_algebra(nd, nc, dims, 0.5, 314)
+    g = klft.GaugeField(nd, nc, dims, storage).upload(U)
+    if sym:
+        g.set_proposal("symmetric")
+    p = klft.MetropolisParams(nHits=8, beta=beta, delta=delta, seed=2718)
+    Uo = U.copy()
+    counts = []
+    for s in range(2):
+        n_g = klft.sweep_Metropolis(g, p, s, with_count=True)[1]
+        n_o = orc.metropolis_sweep_narrow(Uo, nd, nc, dims, 8, beta, delta, 2718, s, sym=sym)[1]
+        counts.append((n_g, n_o))
+    d = np.abs(g.download() - Uo).reshape(-1, nc * nc).max(axis=1)
+    agree = d < 1e-4
+    frac = float(agree.mean())
+    err = float(d[agree].max())
+    print(f"FP32 Metropolis {storage} nd={nd} nc={nc} sym={sym}: counts (GPU, oracle) {counts}, "
+          f"links agreeing {frac:.4%}, max |dU| on them {err:.2e}")
+    for n_g, n_o in counts:
+        assert abs(n_g - n_o) <= max(2, 0.005 * n_o), counts
+    assert frac >= 0.97, frac
+    assert err < 5e-6, err

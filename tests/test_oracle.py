@@ -302,6 +302,38 @@ def test_metropolis_2d_exact_plaquette(nc, beta, exact):
     assert abs(m - exact(beta)) < 4 * e + 2e-3, (m, e, exact(beta))
 
 
+def test_narrow_draws_are_the_fp32_uniforms():
+    """The oracle's FP32-draw sweep takes every uniform as 24 bits of one
+    stream word: with nHits = 1 and delta = 0 (r = 1, dS = 0, always
+    accepted) nothing depends on the draws, and with a U(1) proposal the
+    phase of the one hit is delta * pi * (2 u - 1) for u the word's top 24
+    bits -- recomputed here from the raw Philox words."""
+    nd, nc, dims = 2, 1, (2, 2)
+    U = orc.identity(nd, nc, dims)
+    orc.metropolis_sweep_narrow(U, nd, nc, dims, 1, 0.0, 0.3, 99, 0)
+    # beta = 0: every hit accepted, U = r; the first site (0,0) mu = 0
+    # stream (tag 1, site 0, sweep 0, hit 0, chain 0, mu 0), block 0
+    w = orc.philox([0, 0, 0, 1 << 28], [99, 0])
+    u = (int(w[0]) >> 8) / 16777216.0
+    th = -0.3 * np.pi + 0.6 * np.pi * u
+    assert abs(U[0, 0, 0, 0, 0] - np.exp(1j * th)) < 1e-15
+
+
+@pytest.mark.parametrize("nc,beta,exact", [(1, 2.0, lambda b: iv(1, b) / iv(0, b)),
+                                           (2, 2.0, lambda b: iv(2, b) / iv(1, b))])
+def test_metropolis_2d_exact_plaquette_narrow_draws(nc, beta, exact):
+    """The FP32 links' draws (24-bit uniforms) sample the same 2D ensembles."""
+    nd, dims = 2, (16, 16)
+    U = orc.identity(nd, nc, dims)
+    meas = []
+    for s in range(600):
+        orc.metropolis_sweep_narrow(U, nd, nc, dims, 10, beta, 0.5 if nc == 1 else 0.3, 2025, s)
+        if s >= 100:
+            meas.append(orc.plaquette(U, nd, nc, dims))
+    m, e = binned_mean(meas)
+    assert abs(m - exact(beta)) < 4 * e + 2e-3, (m, e, exact(beta))
+
+
 # --------------------------------------------------------------------------
 # HMC
 # --------------------------------------------------------------------------
