@@ -846,11 +846,13 @@ double orc_metropolis_sweep_sym(int nd, int nc, const int *dims, double *U, int 
 
 /* The same sweep with the FP32 links' draws: every uniform is 24 bits of one
  * stream word (the library's FP32 draw), so the proposals and accept draws
- * are those of the FP32 kernels, with FP64 arithmetic throughout. */
+ * are those of the FP32 kernels, with FP64 arithmetic throughout.  chain is
+ * the stream's chain index (a batched chain's position in its batch). */
 double orc_metropolis_sweep_narrow(int nd, int nc, const int *dims, double *U, int nhits, double beta,
-                                   double delta, uint64_t seed, uint32_t sweep, int sym, long *naccepted) {
+                                   double delta, uint64_t seed, uint32_t sweep, uint32_t chain, int sym,
+                                   long *naccepted) {
   lat l = mklat(nd, nc, dims);
-  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, 0, naccepted, NULL, sym, 1);
+  return metropolis_impl(&l, U, NULL, nhits, beta, delta, seed, sweep, chain, naccepted, NULL, sym, 1);
 }
 
 int orc_rand_sym3(double delta, uint64_t seed, uint32_t sweep, uint32_t site, uint32_t hit,
@@ -1058,4 +1060,12 @@ double orc_metropolis_sweep_defect(int nd, int nc, const int *dims, double *U, c
                                    uint32_t sweep, uint32_t chain, long *naccepted) {
   lat l = mklat(nd, nc, dims);
   return metropolis_impl(&l, U, C, nhits, beta, delta, seed, sweep, chain, naccepted, NULL, 0, 0);
+}
+
+/* the defect sweep with the FP32 links' draws (orc_metropolis_sweep_narrow) */
+double orc_metropolis_sweep_defect_narrow(int nd, int nc, const int *dims, double *U, const double *C,
+                                          int nhits, double beta, double delta, uint64_t seed,
+                                          uint32_t sweep, uint32_t chain, long *naccepted) {
+  lat l = mklat(nd, nc, dims);
+  return metropolis_impl(&l, U, C, nhits, beta, delta, seed, sweep, chain, naccepted, NULL, 0, 1);
 }

@@ -80,9 +80,11 @@ double orc_metropolis_sweep(int nd, int nc, const int *dims, double *U,
 double orc_metropolis_sweep_sym(int nd, int nc, const int *dims, double *U, int nhits, double beta,
                                 double delta, uint64_t seed, uint32_t sweep, long *naccepted);
 /* orc_metropolis_sweep with the FP32 links' draws (24-bit uniforms, one
- * stream word each); sym != 0 draws the symmetric SU(3) proposal */
+ * stream word each) and the stream's chain index; sym != 0 draws the
+ * symmetric SU(3) proposal */
 double orc_metropolis_sweep_narrow(int nd, int nc, const int *dims, double *U, int nhits, double beta,
-                                   double delta, uint64_t seed, uint32_t sweep, int sym, long *naccepted);
+                                   double delta, uint64_t seed, uint32_t sweep, uint32_t chain, int sym,
+                                   long *naccepted);
 int orc_rand_sym3(double delta, uint64_t seed, uint32_t sweep, uint32_t site, uint32_t hit,
                   uint32_t mu, double *out);
 
@@ -116,6 +118,10 @@ double orc_metropolis_sweep_defect(int nd, int nc, const int *dims, double *U,
                                    const double *C, int nhits, double beta,
                                    double delta, uint64_t seed, uint32_t sweep,
                                    uint32_t chain, long *naccepted);
+/* the same with the FP32 links' draws (24-bit uniforms, one stream word each) */
+double orc_metropolis_sweep_defect_narrow(int nd, int nc, const int *dims, double *U, const double *C,
+                                          int nhits, double beta, double delta, uint64_t seed,
+                                          uint32_t sweep, uint32_t chain, long *naccepted);
 
 #ifdef __cplusplus
 }

@@ -174,17 +174,18 @@ def metropolis_sweep_sym(U, nd, nc, dims, nhits, beta, delta, seed, sweep):
 
 
 L.orc_metropolis_sweep_narrow.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_double,
-                                          C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_long)]
+                                          C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_long)]
 L.orc_metropolis_sweep_narrow.restype = C.c_double
 
 
-def metropolis_sweep_narrow(U, nd, nc, dims, nhits, beta, delta, seed, sweep, sym=False):
+def metropolis_sweep_narrow(U, nd, nc, dims, nhits, beta, delta, seed, sweep, sym=False, chain=0):
     """Metropolis sweep with the FP32 links' draws (24-bit uniforms, one stream
-    word each) in FP64 arithmetic, in place on U; returns (acc, naccepted)."""
+    word each) in FP64 arithmetic, in place on U; returns (acc, naccepted).
+    chain: the stream's chain index (a batched chain's position)."""
     assert U.flags["C_CONTIGUOUS"] and U.dtype == np.complex128
     n = C.c_long()
-    acc = L.orc_metropolis_sweep_narrow(nd, nc, _i(dims), _d(U), nhits, beta, delta, seed, sweep, int(sym),
-                                        C.byref(n))
+    acc = L.orc_metropolis_sweep_narrow(nd, nc, _i(dims), _d(U), nhits, beta, delta, seed, sweep, chain,
+                                        int(sym), C.byref(n))
     return acc, n.value
 
 
@@ -295,8 +296,14 @@ def plaquette_defect(U, Cf, nd, nc, dims, normalize=True):
                                   _d(np.ascontiguousarray(Cf)), int(normalize))
 
 
-def metropolis_sweep_defect(U, Cf, nd, nc, dims, nhits, beta, delta, seed, sweep, chain):
+L.orc_metropolis_sweep_defect_narrow.argtypes = L.orc_metropolis_sweep_defect.argtypes
+L.orc_metropolis_sweep_defect_narrow.restype = C.c_double
+
+
+def metropolis_sweep_defect(U, Cf, nd, nc, dims, nhits, beta, delta, seed, sweep, chain, narrow=False):
+    """narrow: the FP32 links' draws (24-bit uniforms, one stream word each)."""
     n = C.c_long()
-    acc = L.orc_metropolis_sweep_defect(nd, nc, _i(dims), _d(U), _d(np.ascontiguousarray(Cf)), nhits,
-                                        beta, delta, seed, sweep, chain, C.byref(n))
+    fn = L.orc_metropolis_sweep_defect_narrow if narrow else L.orc_metropolis_sweep_defect
+    acc = fn(nd, nc, _i(dims), _d(U), _d(np.ascontiguousarray(Cf)), nhits, beta, delta, seed, sweep, chain,
+             C.byref(n))
     return acc, n.value

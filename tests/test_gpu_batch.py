@@ -123,6 +123,53 @@ def test_ptbc_fp64_step_matches_oracle(monkeypatch, variant, dims):
             assert abs(p.action(r, k) - want) < 1e-9 * abs(want)
 
 
+def test_ptbc_fp32_step_matches_oracle_with_fp32_draws():
+    """C5's setting at 8^4 (SU(3), 2^3 defect, c_r linear from 1 to 0, nHits
+    = 8, delta = 0.2) in FP32: each replica's sweep takes the decisions of the
+    oracle run with the FP32 draws and the replica's stream index (links that
+    agree to 1e-4: at least 97 %, within 5e-6 on those), and the FP64 swap
+    pass on the FP32 fields takes the oracle's swap decisions (Metropolis.hpp
+    :72-113 with the defect scaling of PTBC)."""
+    nrep, nc, beta, dims = 4, 3, 6.0, (8, 8, 8, 8)
+    L, pos = 2, [0, 0, 0]
+    c = np.linspace(1.0, 0.0, nrep)
+    seed = 5150
+    p = klft.PTBC(nrep, nc, dims, beta, L, pos, c, "f32")
+    refs = [orc.random_algebra(4, nc, dims, 0.3, 80 + r) for r in range(nrep)]
+    for r in range(nrep):
+        p.upload(r, refs[r])
+    conf_of = list(range(nrep))
+    worst_frac, worst_err = 1.0, 0.0
+    for step in range(3):
+        swaps = p.step(8, 0.2, seed, step)
+        cval = {conf_of[k]: c[k] for k in range(nrep)}
+        for r in range(nrep):
+            C = orc.defect_field(4, dims, L, pos, cval[r])
+            orc.metropolis_sweep_defect(refs[r], C, 4, nc, dims, 8, beta, 0.2, seed, step, r, narrow=True)
+            d = np.abs(p.download(r) - refs[r]).reshape(-1, nc * nc).max(axis=1)
+            agree = d < 1e-4
+            worst_frac = min(worst_frac, float(agree.mean()))
+            worst_err = max(worst_err, float(d[agree].max()))
+
+        def S(U, ck):
+            return -(beta / nc) * orc.plaquette_defect(U, orc.defect_field(4, dims, L, pos, ck), 4, nc, dims,
+                                                       normalize=False)
+        want = []
+        for k in range(nrep - 1):
+            a, b_ = conf_of[k], conf_of[k + 1]
+            dS = S(refs[a], c[k + 1]) + S(refs[b_], c[k]) - S(refs[a], c[k]) - S(refs[b_], c[k + 1])
+            u = orc.stream_doubles(seed, 8, k, step, 0, 0, 0, 1)[0]
+            acc = dS <= 0 or u < np.exp(-dS)
+            want.append(int(acc))
+            if acc:
+                conf_of[k], conf_of[k + 1] = b_, a
+        assert list(swaps) == want, (step, list(swaps), want)
+        assert list(p.labels()) == conf_of
+    print(f"PTBC FP32 vs oracle (FP32 draws): links agreeing >= {worst_frac:.4%}, max |dU| on them "
+          f"{worst_err:.2e}; swap decisions identical")
+    assert worst_frac >= 0.97 and worst_err < 5e-6
+
+
 def test_ptbc_c5_shape_runs_in_fp32():
     """The 16-replica SU(3) 16^4 set: one graph-captured step (sweep of every
     replica, FP64 swap pass) completes and keeps the labels a permutation."""

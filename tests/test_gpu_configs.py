@@ -141,6 +141,36 @@ def test_c4_fp64_batch_matches_oracle_chain_by_chain():
         np.testing.assert_allclose(b.download(i), refs[i], rtol=0, atol=1e-11)
 
 
+def test_c4_fp32_batch_matches_oracle_chain_by_chain():
+    """FP32 links: each of the 64 chains takes the decisions of the oracle run
+    with the FP32 draws (24-bit uniforms) and the chain's stream index, in
+    FP64 arithmetic; accept counts within 0.5 % (at least 2) per chain, links
+    that agree to 1e-4 at least 97 % of each chain's links and within 5e-6 on
+    those (a decision that FP32 rounding moves across its threshold changes
+    that link and its neighbours' later hits)."""
+    chains = c4_chains()
+    b = klft.Batch(chains, "f32")
+    refs = []
+    for i, (nc, dims, _) in enumerate(chains):
+        U = orc.random_algebra(4, nc, dims, 0.3, 900 + i)
+        b.upload(i, U)
+        refs.append(U)
+    acc = b.sweep(10, 0.1, 2026, 0)
+    worst_frac, worst_err, ndiff = 1.0, 0.0, 0
+    for i, (nc, dims, beta) in enumerate(chains):
+        a, n_o = orc.metropolis_sweep_narrow(refs[i], 4, nc, dims, 10, beta, 0.1, 2026, 0, chain=i)
+        n_g = int(round(acc[i] * np.prod(dims) * 4 * 10))
+        ndiff += int(n_g != n_o)
+        assert abs(n_g - n_o) <= max(2, 0.005 * n_o), (i, n_g, n_o)
+        d = np.abs(b.download(i) - refs[i]).reshape(-1, nc * nc).max(axis=1)
+        agree = d < 1e-4
+        worst_frac = min(worst_frac, float(agree.mean()))
+        worst_err = max(worst_err, float(d[agree].max()))
+    print(f"C4 FP32 batch vs oracle (FP32 draws): chains with a differing count {ndiff}/64, "
+          f"links agreeing >= {worst_frac:.4%} per chain, max |dU| on them {worst_err:.2e}")
+    assert worst_frac >= 0.97 and worst_err < 5e-6
+
+
 def test_c4_fp32_batch_tracks_fp64():
     """FP32 links + FP64 plaquette sums against FP64 links, same chains and
     keyed streams, from the cold start: after 100 graph-replayed sweeps, 30
