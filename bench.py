@@ -120,7 +120,8 @@ def hmc_leg(klft, st, a, dims):
 
 def batched_leg(klft, a):
     """The batched engine at the headline's volume: 16 independent SU(3) 16^4
-    chains (1,048,576 sites, as 32^4) with FP32 links + FP64 sums, one launch
+    chains (1,048,576 sites, as 32^4), and at four times it (64 chains), with
+    FP32 links + FP64 sums (and FP64 links for comparison), one launch
     per sublattice pass over all chains, sweeps replayed from one captured
     HIP graph; and the PTBC replica set (16 x SU(3) 16^4, 2^3 defect at
     x0 = 0, c_r = 1 - r/15, nHits = 8, delta = 0.2) stepped with its swap
@@ -128,22 +129,26 @@ def batched_leg(klft, a):
     36 (V/16) B_U)."""
     import numpy as np
     out = {}
-    chains = [(3, (16, 16, 16, 16), a.beta)] * 16
-    vtot = 16 * 16 ** 4
-    for st in ["f32", "f32_rows", "f64"]:
-        b = klft.Batch(chains, st)
-        b.set_identity()
-        b.run(a.nhits, a.delta, 4242, 0, 3)
-        nsw = 20
-        t = sync_time(lambda: b.run(a.nhits, a.delta, 4242, 3, nsw))
-        alg = 36 * vtot * BYTES_PER_LINK[st]
-        out[f"chains16_su3_16^4_{st}"] = {
-            "sweeps_per_s": nsw / t, "link_updates_per_s": 4 * vtot * nsw / t, "ms_per_sweep": 1e3 * t / nsw,
-            "alg_bytes_per_sweep": alg, "achieved_GBs": alg * nsw / t / 1e9,
-            "frac_hbm": alg * nsw / t / 1e9 / PEAK_HBM_GBS}
-        b.close()
-        print(f"[bench] batched 16 x 16^4 {st}: {4 * vtot * nsw / t:.3e} lu/s", file=sys.stderr, flush=True)
+    # 16 chains: the headline's volume; 64 chains: four times it, two blocks
+    # of a pass per CU, the engine with the GPU full
+    for nch in (16, 64):
+        chains = [(3, (16, 16, 16, 16), a.beta)] * nch
+        vtot = nch * 16 ** 4
+        for st in ["f32", "f32_rows", "f64"]:
+            b = klft.Batch(chains, st)
+            b.set_identity()
+            b.run(a.nhits, a.delta, 4242, 0, 3)
+            nsw = 20 if nch == 16 else 8
+            t = sync_time(lambda: b.run(a.nhits, a.delta, 4242, 3, nsw))
+            alg = 36 * vtot * BYTES_PER_LINK[st]
+            out[f"chains{nch}_su3_16^4_{st}"] = {
+                "sweeps_per_s": nsw / t, "link_updates_per_s": 4 * vtot * nsw / t, "ms_per_sweep": 1e3 * t / nsw,
+                "alg_bytes_per_sweep": alg, "achieved_GBs": alg * nsw / t / 1e9,
+                "frac_hbm": alg * nsw / t / 1e9 / PEAK_HBM_GBS}
+            b.close()
+            print(f"[bench] batched {nch} x 16^4 {st}: {4 * vtot * nsw / t:.3e} lu/s", file=sys.stderr, flush=True)
     n = 16
+    vtot = n * 16 ** 4
     p = klft.PTBC(n, 3, (16, 16, 16, 16), 6.0, 2, [0, 0, 0], 1.0 - np.arange(n) / 15.0, "f32")
     p.set_identity()
     for s in range(5):

@@ -93,8 +93,10 @@ __global__ __launch_bounds__(kB) void k_batch_pass(const ChainDev *__restrict__ 
 // block b of the work table holds the 256 positions of chain block_chain[b]
 // that k_batch_pass gives it, in 8 waves.  The group is uniform per block, so
 // all eight waves meet the same barriers.
-template <typename R, bool COMP, bool DEFECT>
-__global__ __launch_bounds__(klft_metro::kWsBlock) void k_batch_ws(
+// FP32 chains (no defect) with the upper-terms-first order fit two blocks per CU (128
+// VGPRs: four waves per SIMD); the register allocator is asked for that
+template <typename R, bool COMP, bool DEFECT, bool ILV = false>
+__global__ __launch_bounds__(klft_metro::kWsBlock, (sizeof(R) == 4 && !ILV && !DEFECT) ? 4 : 2) void k_batch_ws(
     const ChainDev *__restrict__ chains, const int *__restrict__ block_chain, int pass, int nhits, R delta,
     uint64_t seed, const uint32_t *__restrict__ d_sweep, unsigned long long *counters, Defect d,
     const double *__restrict__ cval) {
@@ -110,19 +112,19 @@ __global__ __launch_bounds__(klft_metro::kWsBlock) void k_batch_ws(
   switch (ch.nc) {
     case 1: {
       Links<R, 1, false> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
-      n = klft_metro::ws_pass<R, 1, false, 4, false, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 1.0), delta,
+      n = klft_metro::ws_pass<R, 1, false, 4, ILV, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 1.0), delta,
                                                              seed, sweep, (uint32_t)c, nullptr, d, cc, stage);
       break;
     }
     case 2: {
       Links<R, 2, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
-      n = klft_metro::ws_pass<R, 2, COMP, 4, false, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 2.0), delta,
+      n = klft_metro::ws_pass<R, 2, COMP, 4, ILV, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 2.0), delta,
                                                             seed, sweep, (uint32_t)c, nullptr, d, cc, stage);
       break;
     }
     default: {
       Links<R, 3, COMP> l{reinterpret_cast<cx<R> *>(ch.links), ch.g.V, 4};
-      n = klft_metro::ws_pass<R, 3, COMP, 4, false, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 3.0), delta,
+      n = klft_metro::ws_pass<R, 3, COMP, 4, ILV, DEFECT>(l, ch.g, pass, t, nhits, (R)(ch.beta / 3.0), delta,
                                                             seed, sweep, (uint32_t)c, nullptr, d, cc, stage);
       break;
     }
@@ -327,19 +329,28 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
   auto passes = [&](auto rt, auto ct) -> int {
     using R = typename decltype(rt)::type;
     constexpr bool COMP = decltype(ct)::value != 0;
-    // variant 0: one lane per site (k_batch_pass); 1 (default): site waves
-    // fed by staple waves (k_batch_ws)
-    const bool ws = ki::forced_variant("batch") != 0;
+    // variant 0: one lane per site (k_batch_pass); 1: site waves fed by
+    // staple waves (k_batch_ws), upper staple terms first; 2: the same with
+    // each direction's upper and lower terms back to back.  Default as for a
+    // single field: 2 for FP64 links, and for FP32 links while a pass fills at
+    // most one 512-thread block per CU (2 takes more registers, 1 leaves room
+    // for two blocks per CU)
+    const int fv = ki::forced_variant("batch");
+    const int v = fv >= 0 && fv <= 2 ? fv : ((sizeof(R) == 8 || b->nblocks <= ki::cu_count()) ? 2 : 1);
+    const bool ws = v != 0;
     for (int pass = 0; pass < 16; ++pass) {
-      if (ws && b->ptbc)
-        hipLaunchKernelGGL((k_batch_ws<R, COMP, true>), dim3(b->nblocks), dim3(klft_metro::kWsBlock), 0,
-                           ki::stream(), b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep,
-                           b->d_counters, b->defect, b->d_cval);
-      else if (ws)
-        hipLaunchKernelGGL((k_batch_ws<R, COMP, false>), dim3(b->nblocks), dim3(klft_metro::kWsBlock), 0,
-                           ki::stream(), b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep,
-                           b->d_counters, b->defect, b->d_cval);
-      else if (b->ptbc)
+      auto site_waves = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(b->nblocks), dim3(klft_metro::kWsBlock), 0, ki::stream(), b->d_chains,
+                           b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters, b->defect,
+                           b->d_cval);
+      };
+      if (ws && b->ptbc) {
+        if (v == 2) site_waves(k_batch_ws<R, COMP, true, true>);
+        else site_waves(k_batch_ws<R, COMP, true, false>);
+      } else if (ws) {
+        if (v == 2) site_waves(k_batch_ws<R, COMP, false, true>);
+        else site_waves(k_batch_ws<R, COMP, false, false>);
+      } else if (b->ptbc)
         hipLaunchKernelGGL((k_batch_pass<R, COMP, true>), dim3(b->nblocks), dim3(kB), 0, ki::stream(),
                            b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters,
                            b->defect, b->d_cval);

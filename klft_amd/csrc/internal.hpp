@@ -96,6 +96,9 @@ struct ProfScope {
   }
 };
 
+// compute units of the current device (256 on an MI355X), queried once
+int cu_count();
+
 // ---- autotuner (tuner.hip) ----
 bool tuning_enabled();
 std::string tune_key(const char *kernel, const klft_field_s *f);

@@ -78,13 +78,7 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool tim
   // upper-terms-first order (128 VGPRs) for two: the interleaved order wins
   // while a pass fills at most one block per CU (32^4: 72 against 76 us) and
   // loses once blocks queue for a CU (48^4: 398 against 308 us).
-  static const int ncu = [] {
-    int dev = 0, n = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      return 256;
-    return n > 0 ? n : 256;
-  }();
-  const bool one_round = (f->g.Vh + 255) / 256 <= (long long)ncu;
+  const bool one_round = (f->g.Vh + 255) / 256 <= (long long)cu_count();
   const int dflt = (f->storage == KLFT_STORAGE_F64 || one_round) ? 13 : 12;
   // the symmetric SU(3) proposal runs in the site-wave kernels only
   if (f->proposal == KLFT_PROPOSAL_SYMMETRIC && f->nc == 3) return dflt + 2;

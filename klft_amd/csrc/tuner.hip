@@ -37,6 +37,17 @@ static std::mutex g_tmu;
 static std::map<std::string, TuneEntry> g_table;
 static int g_tuning = -1;
 
+int cu_count() {
+  static const int n = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      return 256;
+    return v;
+  }();
+  return n;
+}
+
 bool tuning_enabled() {
   if (g_tuning < 0) {
     const char *e = std::getenv("KLFT_TUNING");

@@ -21,10 +21,11 @@ def start(nc, dims, i):
     return orc.random_algebra(4, nc, dims, 0.4, 100 + i)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_batch_fp64_matches_oracle_per_chain(monkeypatch, variant):
-    """Both pass kernels (0: one lane per site, 1: site waves fed by staple
-    waves) against the oracle chain by chain."""
+    """The pass kernels (0: one lane per site, 1: site waves fed by staple
+    waves, 2: the same with each direction's upper and lower staple terms back
+    to back) against the oracle chain by chain."""
     monkeypatch.setenv("KLFT_VARIANT", f"batch={variant}")
     b = klft.Batch(CHAINS, "f64")
     for i, (nc, dims, _) in enumerate(CHAINS):
@@ -42,7 +43,7 @@ def test_batch_fp64_matches_oracle_per_chain(monkeypatch, variant):
                                                for i, (nc, dims, _) in enumerate(CHAINS)], atol=1e-13)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 def test_batch_graph_replay_equals_direct_sweeps(monkeypatch, storage, variant):
     monkeypatch.setenv("KLFT_VARIANT", f"batch={variant}")
@@ -79,7 +80,7 @@ def test_batch_fp32_plaquette_tracks_fp64():
     assert np.all(np.abs(p32.mean(axis=0) - p64.mean(axis=0)) < 5 * err)
 
 
-@pytest.mark.parametrize("variant,dims", [(0, (4, 4, 4, 4)), (1, (4, 4, 4, 4)), (1, (12, 8, 8, 8))])
+@pytest.mark.parametrize("variant,dims", [(0, (4, 4, 4, 4)), (1, (4, 4, 4, 4)), (1, (12, 8, 8, 8)), (2, (12, 8, 8, 8))])
 def test_ptbc_fp64_step_matches_oracle(monkeypatch, variant, dims):
     """Replica sweeps and swaps against the oracle, for both pass kernels; the
     12 x 8^3 replicas take two blocks per pass, the second partly filled."""
