@@ -368,8 +368,24 @@ extern "C" int klft_field_destroy(klft_field_t f) {
   if (f->partials) (void)hipFree(f->partials);
   if (f->host_result) (void)hipHostFree(f->host_result);
   if (f->run_counters) (void)hipFree(f->run_counters);
+  if (f->meas_ws) (void)hipFree(f->meas_ws);
   f->links = nullptr;
   delete f;
+  return KLFT_OK;
+}
+
+int klft_internal::meas_workspace(klft_field_t f, size_t bytes, void **out) {
+  if (bytes > f->meas_capacity) {
+    if (f->meas_ws) {
+      KLFT_HIP(hipStreamSynchronize(ki::stream()));
+      (void)hipFree(f->meas_ws);
+      f->meas_ws = nullptr;
+      f->meas_capacity = 0;
+    }
+    KLFT_HIP(hipMalloc(&f->meas_ws, bytes));
+    f->meas_capacity = bytes;
+  }
+  *out = f->meas_ws;
   return KLFT_OK;
 }
 
@@ -985,9 +1001,19 @@ int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, 
   double *dpart = nullptr, *dout = nullptr, *dps = nullptr;
   int *dLs = nullptr;
   const int nw = block / 64;  // one partial per wave
-  KLFT_HIP(hipMalloc(&dpart, sizeof(double) * nout * nbx * nw));
-  KLFT_HIP(hipMalloc(&dout, sizeof(double) * nout));
-  KLFT_HIP(hipMalloc(&dLs, sizeof(int) * nL));
+  {
+    // partials, outputs and extents in the field's measurement workspace
+    // (256-byte aligned pieces); the per-site output, a V-sized debug
+    // array, is allocated per call
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t bpart = up(sizeof(double) * nout * nbx * nw), bout = up(sizeof(double) * nout);
+    void *ws = nullptr;
+    const int wrc = ki::meas_workspace(f, bpart + bout + up(sizeof(int) * nL), &ws);
+    if (wrc) return wrc;
+    dpart = static_cast<double *>(ws);
+    dout = reinterpret_cast<double *>(static_cast<char *>(ws) + bpart);
+    dLs = reinterpret_cast<int *>(static_cast<char *>(ws) + bpart + bout);
+  }
   if (per_site_host) KLFT_HIP(hipMalloc(&dps, sizeof(double) * f->g.V));
   KLFT_HIP(hipMemcpyAsync(dLs, Ls.data(), sizeof(int) * nL, hipMemcpyHostToDevice, ki::stream()));
   int ps_li = 0, ps_T = 0;
@@ -1024,9 +1050,6 @@ int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, 
         }
     }
   }
-  (void)hipFree(dpart);
-  (void)hipFree(dout);
-  (void)hipFree(dLs);
   if (dps) (void)hipFree(dps);
   return rc;
 }

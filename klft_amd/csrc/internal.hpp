@@ -34,6 +34,10 @@ struct klft_field_s {
   // kept until the field is destroyed)
   unsigned long long *run_counters = nullptr;
   int run_capacity = 0;
+  // measurement workspace (Wilson-loop partials, outputs, extents; grown on
+  // demand, kept until the field is destroyed)
+  void *meas_ws = nullptr;
+  size_t meas_capacity = 0;
   int proposal = KLFT_PROPOSAL_REFERENCE;  // SU(3) Metropolis proposal law
 };
 
@@ -55,6 +59,9 @@ int hip_fail(hipError_t e, const char *what);
   } while (0)
 
 kd::Geom make_geom(int nd, const int *dims);
+// the field's measurement workspace, at least `bytes` long (grown on demand;
+// growing synchronises the library stream before the old buffer is freed)
+int meas_workspace(klft_field_t f, size_t bytes, void **out);
 // enqueue-only helpers (no host synchronisation)
 int plaquette_raw(klft_field_t f, int slot, double *dev_per_site);  // sum into f->result[slot]
 int read_results(klft_field_t f, int first, int n, double *out);    // synchronises
