@@ -865,14 +865,17 @@ extern "C" int klft_staple_field(klft_field_t f, klft_field_t out) {
 // materialised in HBM.  Wave partial sums go to
 // partials[(((plane * nL + li) * (Tmax+1) + T) * gridDim.x + blockIdx.x)
 //          * waves per block + wave].
-template <typename R, int NC, bool COMP, int ND>
+template <typename R, int NC, bool COMP, int ND, bool LNU = true>
 __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, int mu0, int nu,
                                                  int ncolblk, int nL, const int *__restrict__ Ls,
                                                  int Tmax, double *partials, double *per_site,
                                                  int ps_li, int ps_T) {
   f.nd = ND;
   extern __shared__ __align__(16) unsigned char smem[];
+  // lds: the threads' current mu-lines; lnu (LNU): the columns' nu-links
+  // U_nu(x), read by every step T of the staple recurrence (else loaded)
   Mat<R, NC> *lds = reinterpret_cast<Mat<R, NC> *>(smem);
+  Mat<R, NC> *lnu = lds + blockDim.x;
   const int plane = blockIdx.y;
   const int mu = mu0 < 0 ? plane : mu0;
   const int Ln = g.L[nu];
@@ -905,6 +908,9 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
   for (int d = 0; d < ND; ++d) base += dterm(d, x[d]);
   const long long base_nomu = base - dterm(mu, x[mu]);  // x without its mu term
   const long long term_nu = dterm(nu, x[nu]);
+  if constexpr (LNU) {
+    if (valid) lnu[tid] = f.load(base, nu);
+  }
   Mat<R, NC> F = identity<R, NC>();
   int Lcur = 0;
   for (int li = 0; li < nL; ++li) {
@@ -925,9 +931,14 @@ __global__ __launch_bounds__(512) void k_wilson(Links<R, NC, COMP> f, Geom g, in
       double w = 0.0;
       if (valid) {
         if (T > 0) {
-          const long long tnu = dterm(nu, wrap(k + T - 1, Ln));
-          const Mat<R, NC> a = f.load(base - term_nu + tnu, nu);  // U_nu(x + (T-1) nu)
-          const Mat<R, NC> b = f.load(base_L + tnu, nu);          // U_nu(x + L mu + (T-1) nu)
+          const int kp = wrap(k + T - 1, Ln);
+          const long long tnu = dterm(nu, kp);
+          const Mat<R, NC> b = f.load(base_L + tnu, nu);   // U_nu(x + L mu + (T-1) nu)
+          Mat<R, NC> a;                                     // U_nu(x + (T-1) nu), same column
+          if constexpr (LNU)
+            a = lnu[cl * Ln + kp];
+          else
+            a = f.load(base - term_nu + tnu, nu);
           D = mul(mul_da(a, D), b);
         }
         const int kt = wrap(k + T, Ln);
@@ -993,6 +1004,14 @@ int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, 
   int C = std::max(1, 256 / Ln);
   // whole waves: the trailing threads (cl >= C) hold no column
   int block = ((C * Ln + 63) / 64) * 64;
+  // the block's mu-lines and nu-links in LDS, 2 * block matrices: fewer
+  // columns per block where that does not fit; a column too long for both
+  // keeps only the mu-lines (the nu-links are loaded at every step)
+  while (C > 1 && 2 * block * mbytes > 64 * 1024) {
+    C /= 2;
+    block = ((C * Ln + 63) / 64) * 64;
+  }
+  const bool lnu = 2 * block * mbytes <= 64 * 1024;
   if (block > 512 || block * mbytes > 64 * 1024)
     return ki::set_error(KLFT_E_ARG, "extent %d along nu too large for the Wilson-loop kernel", Ln);
   const long long ncol = f->g.V / Ln;
@@ -1025,9 +1044,14 @@ int wilson_sums(klft_field_t f, int mu0, int nplanes, int nu, const int *pairs, 
   int rc = ki::dispatch(f->storage, f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
     ki::ProfScope prof("wilson", "WilsonLoop_GaugeField_WLoop_munu");
-    hipLaunchKernelGGL((k_wilson<R, NC, COMP, ND>), dim3(nbx, nplanes), dim3(block),
-                       block * sizeof(Mat<R, NC>), ki::stream(), ki::links_of<R, NC, COMP>(f), f->g,
-                       nplanes > 1 ? -1 : mu0, nu, C, nL, dLs, Tmax, dpart, dps, ps_li, ps_T);
+    if (lnu)
+      hipLaunchKernelGGL((k_wilson<R, NC, COMP, ND, true>), dim3(nbx, nplanes), dim3(block),
+                         2 * block * sizeof(Mat<R, NC>), ki::stream(), ki::links_of<R, NC, COMP>(f), f->g,
+                         nplanes > 1 ? -1 : mu0, nu, C, nL, dLs, Tmax, dpart, dps, ps_li, ps_T);
+    else
+      hipLaunchKernelGGL((k_wilson<R, NC, COMP, ND, false>), dim3(nbx, nplanes), dim3(block),
+                         block * sizeof(Mat<R, NC>), ki::stream(), ki::links_of<R, NC, COMP>(f), f->g,
+                         nplanes > 1 ? -1 : mu0, nu, C, nL, dLs, Tmax, dpart, dps, ps_li, ps_T);
     KLFT_HIP(hipGetLastError());
     return KLFT_OK;
   });

@@ -149,6 +149,20 @@ def test_wilson_temporal_parity(nc):
         assert abs(w - r) <= 1e-12
 
 
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_wilson_temporal_long_time_extent(storage, tol):
+    """A time extent of 256: one FP64 column's mu-lines and nu-links do not
+    both fit the kernel's LDS, so the nu-links are loaded at every step (FP32
+    columns still stage them); both against the oracle."""
+    nd, nc, dims = 4, 3, (4, 2, 2, 256)
+    U = seeded(nd, nc, dims, 0.4, 8)
+    pairs = [(1, 1), (2, 3), (3, 8), (4, 5)]
+    ref = orc.wilson_temporal(U, nd, nc, dims, pairs)
+    g = klft.GaugeField(nd, nc, dims, storage).upload(U)
+    for (L, T, w), r in zip(klft.WilsonLoop_temporal(g, pairs), ref):
+        assert abs(w - r) <= tol, (L, T, w, r)
+
+
 def test_wilson_cold_is_one():
     g = klft.GaugeField(4, 3, (4, 4, 4, 8), "f64").set_identity()
     for L, T, w in klft.WilsonLoop_temporal(g, [(1, 1), (2, 3), (4, 8)]):
