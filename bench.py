@@ -78,15 +78,21 @@ def metropolis_leg(klft, st, a, dims):
     klft.profile(True)
     klft.metropolis_run(g, p, a.warmup + a.steps, nprof, rates=False)
     ms, n = klft.profile_read("metropolis_pass")
+    # launches that run two independent passes at once (when both fit)
+    ms2, n2 = klft.profile_read("metropolis_pass_pair")
     klft.profile(False)
     acc = klft.sweep_Metropolis(g, p, a.warmup + a.steps + nprof)
     plaq = klft.GaugePlaquette(g)
     lu = 4 * V * a.steps / t
     pass_us = 1e3 * ms / max(n, 1)
+    sweep_kernel_us = 1e3 * (ms + ms2) / nprof
     alg = 36 * (V // 16) * BYTES_PER_LINK[st]
-    return g, {"link_updates_per_s": lu, "ms_per_sweep": 1e3 * t / a.steps, "pass_us": pass_us,
-               "pass_launches": n, "alg_bytes_per_pass": alg,
-               "achieved_GBs": alg / (pass_us * 1e-6) / 1e9, "acceptance": acc, "plaquette": plaq}
+    out = {"link_updates_per_s": lu, "ms_per_sweep": 1e3 * t / a.steps, "pass_us": pass_us,
+           "pass_launches": n, "alg_bytes_per_pass": alg,
+           "achieved_GBs": 16 * alg / (sweep_kernel_us * 1e-6) / 1e9, "acceptance": acc, "plaquette": plaq}
+    if n2:
+        out.update({"pair_us": 1e3 * ms2 / n2, "pair_launches": n2, "sweep_kernel_us": sweep_kernel_us})
+    return g, out
 
 
 def hmc_leg(klft, st, a, dims):

@@ -54,7 +54,8 @@ const char *klft_version(void);
 /* ---- gauge fields (replaces deviceGaugeField<Nd,Nc>, GaugeField.hpp:30-239,
  *      3D :241-435, 2D :437-620) ---------------------------------------------- */
 /* nd in {2,3,4}; nc in {1,2,3}; every extent even and >= 2 (the checkerboard
- * of Metropolis.hpp:127-130 silently skips a slice for odd extents). */
+ * of Metropolis.hpp:127-130 silently skips a slice for odd extents).  The
+ * links start zeroed: a set_* call or an upload gives the start. */
 int klft_field_create(int nd, int nc, const int *dims, int storage, klft_field_t *out);
 int klft_field_destroy(klft_field_t f);
 int klft_field_info(klft_field_t f, int *nd, int *nc, int *dims, int *storage);

@@ -98,14 +98,21 @@ __global__ __launch_bounds__(kB) void k_batch_pass(const ChainDev *__restrict__ 
 // for that
 template <typename R, bool COMP, bool DEFECT, bool ILV = false>
 __global__ __launch_bounds__(klft_metro::kWsBlock, (sizeof(R) == 4 && !ILV && !DEFECT) ? 4 : 2) void k_batch_ws(
-    const ChainDev *__restrict__ chains, const int *__restrict__ block_chain, int pass, int nhits, R delta,
-    uint64_t seed, const uint32_t *__restrict__ d_sweep, unsigned long long *counters, Defect d,
+    const ChainDev *__restrict__ chains, const int *__restrict__ block_chain, int pass, int pass2, int nhits,
+    R delta, uint64_t seed, const uint32_t *__restrict__ d_sweep, unsigned long long *counters, Defect d,
     const double *__restrict__ cval) {
   __shared__ cx<R> stage[4 * 4 * 9 * 64];
   __shared__ unsigned long long red[klft_metro::kWsBlock / 64];
-  const int c = block_chain[blockIdx.x];
+  // pass2 >= 0: two independent passes in one launch (pass_groups), the
+  // second half of the grid taking pass2
+  unsigned int bid = blockIdx.x;
+  if (pass2 >= 0 && bid >= gridDim.x / 2) {
+    pass = pass2;
+    bid -= gridDim.x / 2;
+  }
+  const int c = block_chain[bid];
   const ChainDev ch = chains[c];
-  const long long t = (long long)(blockIdx.x - ch.first_block) * kB + (long long)((threadIdx.x >> 6) & 3) * 64 +
+  const long long t = (long long)((long long)bid - ch.first_block) * kB + (long long)((threadIdx.x >> 6) & 3) * 64 +
                       (threadIdx.x & 63);
   const uint32_t sweep = *d_sweep;
   const R cc = DEFECT ? (R)cval[c] : R(1);
@@ -324,6 +331,34 @@ int build_tables(klft_batch_t b) {
   return KLFT_OK;
 }
 
+// the site-wave kernel of variant v (1, 2) for chains or replicas
+template <typename R, bool COMP>
+static auto batch_ws_kernel(bool ptbc, int v) {
+  auto k = &k_batch_ws<R, COMP, false, false>;
+  if (ptbc)
+    k = v == 2 ? &k_batch_ws<R, COMP, true, true> : &k_batch_ws<R, COMP, true, false>;
+  else if (v == 2)
+    k = &k_batch_ws<R, COMP, false, true>;
+  return k;
+}
+
+// the occupancy queries behind the pairing decision, made (and cached)
+// before a graph capture begins, so that the capture sees stream work only
+static void prime_occupancy(klft_batch_t b) {
+  auto prime = [&](auto rt, auto ct) {
+    using R = typename decltype(rt)::type;
+    constexpr bool COMP = decltype(ct)::value != 0;
+    for (int v = 1; v <= 2; ++v)
+      (void)ki::blocks_per_cu(reinterpret_cast<const void *>(batch_ws_kernel<R, COMP>(b->ptbc, v)),
+                              klft_metro::kWsBlock);
+  };
+  switch (b->storage) {
+    case KLFT_STORAGE_F64: prime(ki::tag<double>{}, ki::ic<0>{}); break;
+    case KLFT_STORAGE_F32: prime(ki::tag<float>{}, ki::ic<0>{}); break;
+    default: prime(ki::tag<float>{}, ki::ic<1>{}); break;
+  }
+}
+
 // enqueue one sweep (and, for replicas, the swap pass); no host sync
 int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, ki::stream(), b->d_counters, b->n);
@@ -333,32 +368,41 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
     // variant 0: one lane per site (k_batch_pass); 1: site waves fed by
     // staple waves (k_batch_ws), upper staple terms first; 2: the same with
     // each direction's upper and lower terms back to back.  Default as for a
-    // single field: 2 for FP64 links, and for FP32 links while a pass fills at
-    // most one 512-thread block per CU (2 takes more registers, 1 leaves room
-    // for two blocks per CU)
+    // single field: 2 for FP64 links, and for FP32 links while two paired
+    // passes fill at most one 512-thread block per CU (2 takes more
+    // registers, 1 leaves room for two blocks per CU)
     const int fv = ki::forced_variant("batch");
-    const int v = fv >= 0 && fv <= 2 ? fv : ((sizeof(R) == 8 || b->nblocks <= ki::cu_count()) ? 2 : 1);
+    const int v = fv >= 0 && fv <= 2 ? fv : ((sizeof(R) == 8 || 2 * b->nblocks <= ki::cu_count()) ? 2 : 1);
     const bool ws = v != 0;
-    for (int pass = 0; pass < 16; ++pass) {
-      auto site_waves = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(b->nblocks), dim3(klft_metro::kWsBlock), 0, ki::stream(), b->d_chains,
-                           b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters, b->defect,
-                           b->d_cval);
-      };
-      if (ws && b->ptbc) {
-        if (v == 2) site_waves(k_batch_ws<R, COMP, true, true>);
-        else site_waves(k_batch_ws<R, COMP, true, false>);
-      } else if (ws) {
-        if (v == 2) site_waves(k_batch_ws<R, COMP, false, true>);
-        else site_waves(k_batch_ws<R, COMP, false, false>);
-      } else if (b->ptbc)
-        hipLaunchKernelGGL((k_batch_pass<R, COMP, true>), dim3(b->nblocks), dim3(kB), 0, ki::stream(),
-                           b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters,
-                           b->defect, b->d_cval);
-      else
-        hipLaunchKernelGGL((k_batch_pass<R, COMP, false>), dim3(b->nblocks), dim3(kB), 0, ki::stream(),
-                           b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters,
-                           b->defect, b->d_cval);
+    auto site_waves = [&](auto kern, int pa, int pb) {
+      hipLaunchKernelGGL(kern, dim3(pb >= 0 ? 2 * b->nblocks : b->nblocks), dim3(klft_metro::kWsBlock), 0,
+                         ki::stream(), b->d_chains, b->d_block_chain, pa, pb, nhits, (R)delta, seed, b->d_sweep,
+                         b->d_counters, b->defect, b->d_cval);
+    };
+    auto ws_kernel = [&]() { return batch_ws_kernel<R, COMP>(b->ptbc, v); };
+    // independent passes two to a launch (bit-identical to one by one, see
+    // pass_groups) when both passes' blocks fit on the GPU at once
+    // (KLFT_VARIANT=batch_pairs=0: never)
+    const bool pair = ws && ki::forced_variant("batch_pairs") != 0 &&
+                      2LL * b->nblocks <= (long long)ki::cu_count() *
+                                              ki::blocks_per_cu(reinterpret_cast<const void *>(ws_kernel()),
+                                                                klft_metro::kWsBlock);
+    if (pair) {
+      const klft_metro::PassGroups gr = klft_metro::pass_groups(4);
+      for (int i = 0; i < gr.n; ++i) site_waves(ws_kernel(), gr.p[i][0], gr.p[i][1]);
+    } else {
+      for (int pass = 0; pass < 16; ++pass) {
+        if (ws)
+          site_waves(ws_kernel(), pass, -1);
+        else if (b->ptbc)
+          hipLaunchKernelGGL((k_batch_pass<R, COMP, true>), dim3(b->nblocks), dim3(kB), 0, ki::stream(),
+                             b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters,
+                             b->defect, b->d_cval);
+        else
+          hipLaunchKernelGGL((k_batch_pass<R, COMP, false>), dim3(b->nblocks), dim3(kB), 0, ki::stream(),
+                             b->d_chains, b->d_block_chain, pass, nhits, (R)delta, seed, b->d_sweep, b->d_counters,
+                             b->defect, b->d_cval);
+      }
     }
     if (b->ptbc) {
       launch_defect_sums<R, COMP>(b->nc, b->d_chains, b->n, b->defect, b->d_A);
@@ -386,6 +430,7 @@ int ensure_graph(klft_batch_t b, int nhits, double delta, uint64_t seed) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
+  prime_occupancy(b);
   KLFT_HIP(hipStreamBeginCapture(ki::stream(), hipStreamCaptureModeThreadLocal));
   const int rc = enqueue_step(b, nhits, delta, seed);
   hipGraph_t g = nullptr;

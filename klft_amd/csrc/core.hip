@@ -350,6 +350,10 @@ extern "C" int klft_field_create(int nd, int nc, const int *dims, int storage, k
   f->result = (double *)(w + pbytes + 256);
   f->counter = (unsigned long long *)(w + pbytes + 256 + 16 * sizeof(double));
   e = hipMemset(w, 0, ws);
+  // the links start zeroed, not as whatever the allocation held: the
+  // reference has no field without an initialiser (GaugeField.hpp:33-62),
+  // and here one of the set_* calls or an upload gives the start
+  if (e == hipSuccess) e = hipMemset(f->links, 0, f->link_bytes);
   if (e == hipSuccess) e = hipHostMalloc((void **)&f->host_result, 16 * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) {
     (void)hipFree(w);

@@ -105,6 +105,9 @@ struct ProfScope {
 
 // compute units of the current device (256 on an MI355X), queried once
 int cu_count();
+// workgroups of `block` threads of kernel `kern` resident per CU at once
+// (the occupancy query, cached per kernel; at least 1)
+int blocks_per_cu(const void *kern, int block);
 
 // ---- autotuner (tuner.hip) ----
 bool tuning_enabled();

@@ -403,19 +403,63 @@ __device__ __forceinline__ void ws_count(unsigned long long n, unsigned long lon
   }
 }
 
+// pass2 >= 0: the launch runs the independent passes `pass` and `pass2`
+// (pass_groups), the odd logical blocks taking pass2, so that the blocks of
+// one XCD hold the same chunk of both sublattices
 template <typename R, int NC, bool COMP, int ND, bool ILV, bool SYM = false>
 __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
-    Links<R, NC, COMP> f, Geom g, int pass, int nhits, R beta_nc, R delta, uint64_t seed,
+    Links<R, NC, COMP> f, Geom g, int pass, int pass2, int nhits, R beta_nc, R delta, uint64_t seed,
     uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
   f.nd = ND;
   __shared__ cx<R> stage[4 * 4 * NC * NC * 64];
   __shared__ unsigned long long red[kWsBlock / 64];
-  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  if (pass2 >= 0) {
+    if (lb & 1u) pass = pass2;
+    lb >>= 1;
+  }
   const long long t = (long long)lb * kWsSites + (long long)((threadIdx.x >> 6) & 3) * 64 + (threadIdx.x & 63);
   const unsigned long long n = ws_pass<R, NC, COMP, ND, ILV, false, SYM>(f, g, pass, t, nhits, beta_nc, delta,
                                                                          seed, sweep, chain, visit, Defect{}, R(1),
                                                                          stage);
   ws_count(n, red, nacc);
+}
+
+// Passes that neither read nor write each other's links commute.  A pass
+// writes only its own sites' links, and its staples read the links of the
+// sublattices whose coordinate parities differ from its own in one or two
+// places (x + mu, x - nu, x + mu - nu).  Passes p, q with three or more
+// parities apart are therefore independent, and since every draw is keyed by
+// (site, sweep, direction, hit) and not by launch order, running them in one
+// launch gives the sequential sweep's result bit for bit.  The groups below
+// follow the sweep order 0 .. 2^nd - 1 (Metropolis.hpp:139-162): a pass
+// joins the group after the last earlier pass it depends on.  4D: 16 passes
+// in 11 launches, {3,4} {5,8} {6,9} {7,10} {11,12} paired; 3D: {3,4} paired;
+// 2D: none.  Each group holds at most two passes.
+struct PassGroups {
+  int n = 0;
+  int p[16][2];
+};
+inline PassGroups pass_groups(int nd) {
+  const int np = 1 << nd;
+  int level[16];
+  PassGroups gr;
+  for (int q = 0; q < np; ++q) {
+    int l = 0;
+    for (int r = 0; r < q; ++r) {
+      const int h = __builtin_popcount(q ^ r);
+      if (h <= 2 && level[r] + 1 > l) l = level[r] + 1;
+    }
+    level[q] = l;
+    if (l == gr.n) {
+      gr.p[gr.n][0] = q;
+      gr.p[gr.n][1] = -1;
+      ++gr.n;
+    } else {
+      gr.p[l][1] = q;  // the level's second pass (a third never occurs for nd <= 4)
+    }
+  }
+  return gr;
 }
 
 // Sweep variants (numbers kept stable for tuning caches): 0 one lane per
@@ -442,42 +486,74 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
       case 0: case 4: case 7: case 11: case 12: case 13: case 14: case 15: break;
       default: return set_error(KLFT_E_ARG, "sweep variant %d is not built", variant);
     }
-    for (int pass = p0; pass < p1; ++pass) {
-      char label[48];
-      snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pass);  // Metropolis.hpp:161
-      ProfScope prof("metropolis_pass", label);
-      if (variant == 14 || variant == 15) {
-        // the symmetric SU(3) proposal: site waves, terms as in 12 / 13
-        if constexpr (NC == 3) {
-          if (variant == 14)
-            hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, false, true>), dim3(blocks_for(f->g.Vh, kWsSites)),
-                               dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta,
-                               p->seed, (uint32_t)sweep, chain, counter, visit);
-          else
-            hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, true, true>), dim3(blocks_for(f->g.Vh, kWsSites)),
-                               dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta,
-                               p->seed, (uint32_t)sweep, chain, counter, visit);
+    const unsigned nws = blocks_for(f->g.Vh, kWsSites);
+    // launch the site-wave kernel on one pass, or on two independent ones
+    auto ws = [&](auto kern, int pa, int pb) {
+      hipLaunchKernelGGL(kern, dim3(pb >= 0 ? 2 * nws : nws), dim3(kWsBlock), 0, stream(), l, f->g, pa, pb,
+                         p->nhits, beta_nc, (R)p->delta, p->seed, (uint32_t)sweep, chain, counter, visit);
+    };
+    auto launch_ws = [&](int pa, int pb) -> int {
+      switch (variant) {
+        case 12: ws(k_metropolis_ws<R, NC, COMP, ND, false>, pa, pb); break;
+        case 13: ws(k_metropolis_ws<R, NC, COMP, ND, true>, pa, pb); break;
+        default:
+          // the symmetric SU(3) proposal: site waves, terms as in 12 / 13
+          if constexpr (NC == 3) {
+            if (variant == 14)
+              ws(k_metropolis_ws<R, NC, COMP, ND, false, true>, pa, pb);
+            else
+              ws(k_metropolis_ws<R, NC, COMP, ND, true, true>, pa, pb);
+          } else {
+            return set_error(KLFT_E_ARG, "the symmetric proposal variants are SU(3) only");
+          }
+      }
+      return KLFT_OK;
+    };
+    // a whole sweep runs its independent passes in pairs when both passes'
+    // blocks fit on the GPU at once (KLFT_VARIANT=metropolis_pairs=0: never)
+    bool pair = false;
+    if (variant >= 12 && p0 == 0 && p1 == (1 << ND) && forced_variant("metropolis_pairs") != 0) {
+      const void *kern = nullptr;
+      switch (variant) {
+        case 12: kern = reinterpret_cast<const void *>(&k_metropolis_ws<R, NC, COMP, ND, false>); break;
+        case 13: kern = reinterpret_cast<const void *>(&k_metropolis_ws<R, NC, COMP, ND, true>); break;
+        case 14: kern = reinterpret_cast<const void *>(&k_metropolis_ws<R, NC, COMP, ND, false, true>); break;
+        default: kern = reinterpret_cast<const void *>(&k_metropolis_ws<R, NC, COMP, ND, true, true>); break;
+      }
+      pair = 2LL * nws <= (long long)cu_count() * blocks_per_cu(kern, kWsBlock);
+    }
+    if (pair) {
+      const PassGroups gr = pass_groups(ND);
+      for (int i = 0; i < gr.n; ++i) {
+        const int pa = gr.p[i][0], pb = gr.p[i][1];
+        char label[56];
+        if (pb >= 0)
+          snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d_%d", pa, pb);
+        else
+          snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pa);  // Metropolis.hpp:161
+        ProfScope prof(pb >= 0 ? "metropolis_pass_pair" : "metropolis_pass", label);
+        const int rc = launch_ws(pa, pb);
+        if (rc) return rc;
+      }
+    } else {
+      for (int pass = p0; pass < p1; ++pass) {
+        char label[48];
+        snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pass);  // Metropolis.hpp:161
+        ProfScope prof("metropolis_pass", label);
+        if (variant >= 12) {
+          const int rc = launch_ws(pass, -1);
+          if (rc) return rc;
         } else {
-          return set_error(KLFT_E_ARG, "the symmetric proposal variants are SU(3) only");
-        }
-      } else if (variant == 12)
-        hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, false>), dim3(blocks_for(f->g.Vh, kWsSites)),
-                           dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
-                           (uint32_t)sweep, chain, counter, visit);
-      else if (variant == 13)
-        hipLaunchKernelGGL((k_metropolis_ws<R, NC, COMP, ND, true>), dim3(blocks_for(f->g.Vh, kWsSites)),
-                           dim3(kWsBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc, (R)p->delta, p->seed,
-                           (uint32_t)sweep, chain, counter, visit);
-      else {
-        auto one_lane = [&](auto kern) {
-          hipLaunchKernelGGL(kern, dim3(nb), dim3(kMetroBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc,
-                             (R)p->delta, p->seed, (uint32_t)sweep, chain, counter, visit);
-        };
-        switch (variant) {
-          case 4: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2>); break;
-          case 7: one_lane(k_metropolis_pipe<R, NC, COMP, ND>); break;
-          case 11: one_lane(k_metropolis_own<R, NC, COMP, ND>); break;
-          default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1>); break;
+          auto one_lane = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(kMetroBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc,
+                               (R)p->delta, p->seed, (uint32_t)sweep, chain, counter, visit);
+          };
+          switch (variant) {
+            case 4: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2>); break;
+            case 7: one_lane(k_metropolis_pipe<R, NC, COMP, ND>); break;
+            case 11: one_lane(k_metropolis_own<R, NC, COMP, ND>); break;
+            default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1>); break;
+          }
         }
       }
     }

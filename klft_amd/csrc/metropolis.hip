@@ -60,8 +60,10 @@ static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p
   return set_error(KLFT_E_ARG, "storage=%d unsupported", f->storage);
 }
 
-// sweep variant for this field; tuning times one pass of each variant on a
-// scratch copy of the links, so the live field is never updated by it
+// sweep variant for this field; tuning times one sweep of each variant on a
+// scratch copy of the links, so the live field is never updated by it (a
+// sweep, not a pass: the site-wave variants pair independent passes when
+// both fit on the GPU at once, which a single pass would not show)
 int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool timed) {
   void *scratch = nullptr;
   // variants (metro_kernels.hpp): 0 one lane per site; 4 one lane with the
@@ -76,10 +78,11 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool tim
   // order for FP64 links.  For FP32 links the interleaved order's six loads
   // in flight (196 VGPRs) leave room for one block per CU, the
   // upper-terms-first order (128 VGPRs) for two: the interleaved order wins
-  // while a pass fills at most one block per CU (32^4: 72 against 76 us) and
-  // loses once blocks queue for a CU (48^4: 398 against 308 us).
-  const bool one_round = (f->g.Vh + 255) / 256 <= (long long)cu_count();
-  const int dflt = (f->storage == KLFT_STORAGE_F64 || one_round) ? 13 : 12;
+  // while two paired passes fill at most one block per CU, and loses once
+  // they need two (32^4: two-block pairs) or blocks queue for a CU (48^4:
+  // 398 against 308 us per pass).
+  const bool paired_one_round = 2 * ((f->g.Vh + 255) / 256) <= (long long)cu_count();
+  const int dflt = (f->storage == KLFT_STORAGE_F64 || paired_one_round) ? 13 : 12;
   // the symmetric SU(3) proposal runs in the site-wave kernels only
   if (f->proposal == KLFT_PROPOSAL_SYMMETRIC && f->nc == 3) return dflt + 2;
   // KLFT_VARIANT="metropolis=v" forces a variant for every hit count
@@ -93,7 +96,7 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool tim
       KLFT_HIP(hipMalloc(&scratch, f->link_bytes));
       KLFT_HIP(hipMemcpyAsync(scratch, f->links, f->link_bytes, hipMemcpyDeviceToDevice, stream()));
     }
-    return launch_passes(f, scratch, var, 0, 1, p, 0, 0, f->counter + 7, nullptr);
+    return launch_passes(f, scratch, var, 0, 1 << f->nd, p, 0, 0, f->counter + 7, nullptr);
   });
   if (scratch) {
     (void)hipStreamSynchronize(stream());

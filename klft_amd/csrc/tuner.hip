@@ -21,6 +21,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "internal.hpp"
@@ -45,6 +46,19 @@ int cu_count() {
       return 256;
     return v;
   }();
+  return n;
+}
+
+int blocks_per_cu(const void *kern, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<const void *, int>, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_pair(kern, block);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, block, 0) != hipSuccess || n < 1) n = 1;
+  cache[key] = n;
   return n;
 }
 

@@ -223,3 +223,41 @@ def test_ptbc_many_replicas_and_full_length_defect(nrep, nc, dims, L, pos):
             conf_of[k], conf_of[k + 1] = b_, a
     assert list(swaps) == want
     assert list(p.labels()) == conf_of
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f32_rows"])
+def test_batch_paired_passes_equal_sequential(monkeypatch, storage, variant):
+    """Independent passes two to a launch (the default while both passes'
+    blocks fit on the GPU) leave every chain's links and accept counts bit for
+    bit as the pass-by-pass sweep does; the same for a PTBC step with its
+    swap decisions."""
+    chains = [(3, (8, 8, 8, 8), 6.0), (2, (8, 6, 4, 10), 2.3), (1, (6, 6, 6, 6), 2.0), (3, (16, 8, 8, 8), 5.5)]
+    if storage == "f32_rows":
+        chains = [c for c in chains if c[0] > 1]
+    out = []
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("KLFT_VARIANT", f"batch={variant},batch_pairs={pairs}")
+        b = klft.Batch(chains, storage)
+        for i, (nc, dims, _) in enumerate(chains):
+            b.upload(i, start(nc, dims, i))
+        acc = [list(b.sweep(10, 0.2, 55, s)) for s in range(2)]
+        b.run(10, 0.2, 55, 2, 2)  # graph-captured sweeps
+        out.append((acc, [b.download(i) for i in range(len(chains))]))
+        b.close()
+    assert out[0][0] == out[1][0]
+    for u0, u1 in zip(out[0][1], out[1][1]):
+        assert np.array_equal(u0, u1)
+    res = []
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("KLFT_VARIANT", f"batch={variant},batch_pairs={pairs}")
+        p = klft.PTBC(6, 3, (8, 8, 8, 8), 5.5, 2, [0, 0, 0], np.linspace(1.0, 0.0, 6),
+                      "f32" if storage == "f32_rows" else storage)
+        for r in range(6):
+            p.upload(r, start(3, (8, 8, 8, 8), 40 + r))
+        sw = [list(p.step(8, 0.2, 9, s)) for s in range(3)]
+        res.append((sw, list(p.labels()), [p.download(i) for i in range(6)]))
+        p.close()
+    assert res[0][0] == res[1][0] and res[0][1] == res[1][1]
+    for u0, u1 in zip(res[0][2], res[1][2]):
+        assert np.array_equal(u0, u1)

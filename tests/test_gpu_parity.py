@@ -320,3 +320,37 @@ def test_removed_variant_is_an_error(monkeypatch):
     g = klft.GaugeField(4, 2, (4, 4, 4, 4), "f64").set_identity()
     with pytest.raises(klft.KlftError, match="not built"):
         klft.sweep_Metropolis(g, klft.MetropolisParams(nHits=2, beta=2.0, delta=0.2, seed=1), 0)
+
+
+@pytest.mark.parametrize("storage,dims,nc,variant", [
+    ("f64", (12, 12, 8, 10), 3, 13), ("f32", (16, 16, 16, 16), 3, 13), ("f32_rows", (16, 16, 16, 16), 3, 12),
+    ("f32", (32, 32, 32, 32), 3, 12), ("f32_rows", (32, 32, 32, 32), 3, 12), ("f64", (10, 8, 12), 3, 12),
+    ("f64", (8, 8, 8, 8), 2, 13), ("f32", (6, 6, 6, 6), 1, 12)])
+def test_paired_passes_equal_sequential_sweep(monkeypatch, storage, dims, nc, variant):
+    """Passes three or more coordinate parities apart read none of each
+    other's links, so a sweep that runs such passes two to a launch (4D:
+    {3,4} {5,8} {6,9} {7,10} {11,12}; 3D: {3,4}) leaves the links and the
+    accept counts bit for bit as the pass-by-pass sweep does.  32^4 FP32 pairs
+    two 512-thread blocks per CU; the smaller lattices one."""
+    nd = len(dims)
+    beta = {1: 2.0, 2: 2.3, 3: 6.0}[nc]
+    p = klft.MetropolisParams(nHits=10, beta=beta, delta=0.1, seed=4242)
+    out = []
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("KLFT_VARIANT", f"metropolis={variant},metropolis_pairs={pairs}")
+        g = klft.GaugeField(nd, nc, dims, storage).set_random_algebra(0.5, 2024)
+        n = [klft.sweep_Metropolis(g, p, s, with_count=True)[1] for s in range(3)]
+        out.append((n, g.download()))
+        g.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f32_rows"])
+def test_new_field_links_start_zeroed(storage):
+    """A field's links read back as zeros before any initialiser runs."""
+    g = klft.GaugeField(4, 3, (4, 4, 4, 6), storage)
+    U = g.download()
+    g.close()
+    assert U.shape == (4, 4, 4, 6, 4, 3, 3)
+    assert not np.any(U)
