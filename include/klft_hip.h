@@ -144,6 +144,13 @@ int klft_metropolis_run(klft_field_t f, const klft_metropolis_params *p, uint64_
  * (lexicographic order, V int32) -- written by the sweep kernel's own
  * thread-to-site map. */
 int klft_metropolis_schedule(klft_field_t f, int32_t *visit);
+/* the launch order of a sweep whose independent passes (three or more
+ * coordinate parities apart) run two to a launch: launch i runs passes
+ * groups[2i] and groups[2i+1] (-1: none), *nlaunches launches in all;
+ * groups holds 2 * 2^nd ints.  No counterpart in the reference, which
+ * launches every pass on its own (Metropolis.hpp:139-162); for callers and
+ * tests that check the schedule.  Host only: no GPU call. */
+int klft_sweep_launches(int nd, int32_t *groups, int *nlaunches);
 
 /* ---- HMC (HMC.hpp:48-71, Integrator.hpp:26-53, GaugeMonomial.hpp:19-37,
  *      HamiltonianField.hpp:23-60) ------------------------------------------------ */

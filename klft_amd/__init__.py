@@ -114,6 +114,7 @@ _SIGS = {
     "klft_metropolis_sweep": (C.c_int, [_p, C.POINTER(MetropolisParamsC), C.c_uint64, _dp, _i64p]),
     "klft_metropolis_run": (C.c_int, [_p, C.POINTER(MetropolisParamsC), C.c_uint64, C.c_int, _dp]),
     "klft_metropolis_schedule": (C.c_int, [_p, _i32p]),
+    "klft_sweep_launches": (C.c_int, [C.c_int, _i32p, C.POINTER(C.c_int)]),
     "klft_field_set_proposal": (C.c_int, [_p, C.c_int]),
     "klft_hmc_create": (C.c_int, [_p, C.POINTER(HmcParamsC), C.POINTER(_p)]),
     "klft_hmc_destroy": (C.c_int, [_p]),
@@ -348,6 +349,15 @@ def metropolis_schedule(g: GaugeField) -> np.ndarray:
     v = np.empty(g.volume, dtype=np.int32)
     _check(lib.klft_metropolis_schedule(g.h, v.ctypes.data_as(_i32p)))
     return v
+
+
+def sweep_launches(nd: int):
+    """The launches of a sweep whose independent passes run two to a launch:
+    a list of pass tuples, in launch order."""
+    g = np.empty(2 << nd, dtype=np.int32)
+    n = C.c_int()
+    _check(lib.klft_sweep_launches(nd, g.ctypes.data_as(_i32p), C.byref(n)))
+    return [tuple(int(p) for p in g[2 * i:2 * i + 2] if p >= 0) for i in range(n.value)]
 
 
 class HMC:

@@ -388,7 +388,7 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
                                               ki::blocks_per_cu(reinterpret_cast<const void *>(ws_kernel()),
                                                                 klft_metro::kWsBlock);
     if (pair) {
-      const klft_metro::PassGroups gr = klft_metro::pass_groups(4);
+      const ki::PassGroups gr = ki::pass_groups(4);
       for (int i = 0; i < gr.n; ++i) site_waves(ws_kernel(), gr.p[i][0], gr.p[i][1]);
     } else {
       for (int pass = 0; pass < 16; ++pass) {

@@ -425,43 +425,6 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
   ws_count(n, red, nacc);
 }
 
-// Passes that neither read nor write each other's links commute.  A pass
-// writes only its own sites' links, and its staples read the links of the
-// sublattices whose coordinate parities differ from its own in one or two
-// places (x + mu, x - nu, x + mu - nu).  Passes p, q with three or more
-// parities apart are therefore independent, and since every draw is keyed by
-// (site, sweep, direction, hit) and not by launch order, running them in one
-// launch gives the sequential sweep's result bit for bit.  The groups below
-// follow the sweep order 0 .. 2^nd - 1 (Metropolis.hpp:139-162): a pass
-// joins the group after the last earlier pass it depends on.  4D: 16 passes
-// in 11 launches, {3,4} {5,8} {6,9} {7,10} {11,12} paired; 3D: {3,4} paired;
-// 2D: none.  Each group holds at most two passes.
-struct PassGroups {
-  int n = 0;
-  int p[16][2];
-};
-inline PassGroups pass_groups(int nd) {
-  const int np = 1 << nd;
-  int level[16];
-  PassGroups gr;
-  for (int q = 0; q < np; ++q) {
-    int l = 0;
-    for (int r = 0; r < q; ++r) {
-      const int h = __builtin_popcount(q ^ r);
-      if (h <= 2 && level[r] + 1 > l) l = level[r] + 1;
-    }
-    level[q] = l;
-    if (l == gr.n) {
-      gr.p[gr.n][0] = q;
-      gr.p[gr.n][1] = -1;
-      ++gr.n;
-    } else {
-      gr.p[l][1] = q;  // the level's second pass (a third never occurs for nd <= 4)
-    }
-  }
-  return gr;
-}
-
 // Sweep variants (numbers kept stable for tuning caches): 0 one lane per
 // site; 4 one lane, low-register staple, >= 2 waves per SIMD; 7 one lane,
 // next direction's staple loaded during the hits; 11 one lane, the site's own

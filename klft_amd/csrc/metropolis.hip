@@ -186,6 +186,18 @@ extern "C" int klft_metropolis_run(klft_field_t f, const klft_metropolis_params 
   return ki::sync_stream();
 }
 
+extern "C" int klft_sweep_launches(int nd, int32_t *groups, int *nlaunches) {
+  if (nd < 2 || nd > 4) return ki::set_error(KLFT_E_ARG, "nd=%d unsupported (2..4)", nd);
+  if (!groups || !nlaunches) return ki::set_error(KLFT_E_ARG, "null argument");
+  const ki::PassGroups gr = ki::pass_groups(nd);
+  for (int i = 0; i < gr.n; ++i) {
+    groups[2 * i] = gr.p[i][0];
+    groups[2 * i + 1] = gr.p[i][1];
+  }
+  *nlaunches = gr.n;
+  return KLFT_OK;
+}
+
 extern "C" int klft_metropolis_schedule(klft_field_t f, int32_t *visit) {
   int rc = ki::check_field(f);
   if (rc) return rc;
