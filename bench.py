@@ -128,7 +128,8 @@ def batched_leg(klft, a):
     """The batched engine at the headline's volume: 16 independent SU(3) 16^4
     chains (1,048,576 sites, as 32^4), and at four times it (64 chains), with
     FP32 links + FP64 sums (and FP64 links for comparison), one launch
-    per sublattice pass over all chains, sweeps replayed from one captured
+    per sublattice pass, or per pair of independent passes, over all
+    chains, sweeps replayed from one captured
     HIP graph; and the PTBC replica set (16 x SU(3) 16^4, 2^3 defect at
     x0 = 0, c_r = 1 - r/15, nHits = 8, delta = 0.2) stepped with its swap
     pass.  Algorithmic bytes per sweep = 36 * V_total * B_U (16 passes of

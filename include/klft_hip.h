@@ -186,7 +186,8 @@ int klft_hmc_trajectory(klft_hmc_t h, uint64_t traj, double *dH, int *accepted);
 /* ---- batched chains and PTBC replicas ---------------------------------------------- */
 /* A batch owns n independent 4D chains of mixed extents, groups U(1)/SU(2)
  * (FP32 links) or SU(3), each with its own beta, updated by one launch per
- * sublattice pass over all chains (captured in a HIP graph). */
+ * sublattice pass (or per pair of independent passes) over all chains
+ * (captured in a HIP graph). */
 typedef struct {
   int nc;        /* 1, 2 or 3 */
   int dims[4];

@@ -1,11 +1,12 @@
 // Batched Metropolis for many independent 4D chains and PTBC replica sets.
 //
-// One launch per sublattice pass updates every chain: the grid is a work
-// table of 256-site blocks, block b belonging to chain block_chain[b], so
-// chains of uneven volume and of different groups (U(1), SU(2), SU(3): the
-// group is uniform per block) and different beta share one dispatch.  Each
-// chain draws from its own keyed streams (chain index in the Philox counter).
-// A sweep of the whole batch -- counter reset, 16 pass launches and, for
+// One launch per sublattice pass (or per pair of independent passes, see
+// pass_groups) updates every chain: the grid is a work table of 256-site
+// blocks, block b belonging to chain block_chain[b], so chains of uneven
+// volume and of different groups (U(1), SU(2), SU(3): the group is uniform
+// per block) and different beta share one dispatch.  Each chain draws from
+// its own keyed streams (chain index in the Philox counter).  A sweep of the
+// whole batch -- counter reset, the 16 passes in 16 or 11 launches and, for
 // replicas, the defect action sums and the swap kernel -- is captured once in
 // a HIP graph and replayed; the sweep / step index lives in device memory and
 // the graph's last node advances it.
