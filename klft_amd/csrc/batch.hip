@@ -94,11 +94,13 @@ __global__ __launch_bounds__(kB) void k_batch_pass(const ChainDev *__restrict__ 
 // block b of the work table holds the 256 positions of chain block_chain[b]
 // that k_batch_pass gives it, in 8 waves.  The group is uniform per block, so
 // all eight waves meet the same barriers.
-// FP32 chains (no defect) with the upper-terms-first order fit two blocks
+// FP32 chains and replicas with the upper-terms-first order fit two blocks
 // per CU (128 VGPRs: four waves per SIMD); the register allocator is asked
-// for that
+// for that (the defect kernels then spill 12 / 24 registers, FP32 two-row /
+// full, and still gain: C5 FP32 917 -> 956 steps/s, two-row 961 -> 1122
+// with their passes paired)
 template <typename R, bool COMP, bool DEFECT, bool ILV = false>
-__global__ __launch_bounds__(klft_metro::kWsBlock, (sizeof(R) == 4 && !ILV && !DEFECT) ? 4 : 2) void k_batch_ws(
+__global__ __launch_bounds__(klft_metro::kWsBlock, (sizeof(R) == 4 && !ILV) ? 4 : 2) void k_batch_ws(
     const ChainDev *__restrict__ chains, const int *__restrict__ block_chain, int pass, int pass2, int nhits,
     R delta, uint64_t seed, const uint32_t *__restrict__ d_sweep, unsigned long long *counters, Defect d,
     const double *__restrict__ cval) {
