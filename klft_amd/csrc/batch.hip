@@ -326,11 +326,16 @@ int build_tables(klft_batch_t b) {
   KLFT_HIP(hipMalloc(&b->d_plaq, sizeof(double) * b->n));
   KLFT_HIP(hipMalloc(&b->d_counters, sizeof(unsigned long long) * b->n));
   KLFT_HIP(hipMalloc(&b->d_sweep, sizeof(uint32_t)));
-  KLFT_HIP(hipMemcpy(b->d_chains, ch.data(), sizeof(ChainDev) * b->n, hipMemcpyHostToDevice));
-  KLFT_HIP(hipMemcpy(b->d_block_chain, bc.data(), sizeof(int) * bc.size(), hipMemcpyHostToDevice));
-  KLFT_HIP(hipMemcpy(b->d_pblock_chain, pbc.data(), sizeof(int) * pbc.size(), hipMemcpyHostToDevice));
-  KLFT_HIP(hipMemset(b->d_counters, 0, sizeof(unsigned long long) * b->n));
-  KLFT_HIP(hipMemset(b->d_sweep, 0, sizeof(uint32_t)));
+  // tables and clears go on the library stream, where the batch kernels run
+  // (a non-blocking stream, not ordered after null-stream work), and are
+  // waited for before the host vectors go out of scope
+  hipStream_t st = ki::stream();
+  KLFT_HIP(hipMemcpyAsync(b->d_chains, ch.data(), sizeof(ChainDev) * b->n, hipMemcpyHostToDevice, st));
+  KLFT_HIP(hipMemcpyAsync(b->d_block_chain, bc.data(), sizeof(int) * bc.size(), hipMemcpyHostToDevice, st));
+  KLFT_HIP(hipMemcpyAsync(b->d_pblock_chain, pbc.data(), sizeof(int) * pbc.size(), hipMemcpyHostToDevice, st));
+  KLFT_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned long long) * b->n, st));
+  KLFT_HIP(hipMemsetAsync(b->d_sweep, 0, sizeof(uint32_t), st));
+  KLFT_HIP(hipStreamSynchronize(st));
   return KLFT_OK;
 }
 
@@ -607,10 +612,14 @@ extern "C" int klft_ptbc_create(int nreplicas, int nc, const int *dims, int stor
   if (e == hipSuccess) e = hipMalloc(&b->d_A, sizeof(double) * 2 * nreplicas);
   if (e == hipSuccess) e = hipMalloc(&b->d_conf_of, sizeof(int) * nreplicas);
   if (e == hipSuccess) e = hipMalloc(&b->d_swaps, sizeof(int) * nreplicas);
-  if (e == hipSuccess) e = hipMemcpy(b->d_cvals, c, sizeof(double) * nreplicas, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(b->d_cval, c, sizeof(double) * nreplicas, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(b->d_conf_of, conf.data(), sizeof(int) * nreplicas, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(b->d_swaps, 0, sizeof(int) * nreplicas);
+  // on the library stream, where the replica kernels run, and waited for
+  hipStream_t st = ki::stream();
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_cvals, c, sizeof(double) * nreplicas, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_cval, c, sizeof(double) * nreplicas, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_conf_of, conf.data(), sizeof(int) * nreplicas, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(b->d_swaps, 0, sizeof(int) * nreplicas, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     klft_batch_destroy(b);
     *out = nullptr;

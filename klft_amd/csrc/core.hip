@@ -349,11 +349,16 @@ extern "C" int klft_field_create(int nd, int nc, const int *dims, int storage, k
   f->ticket = (unsigned int *)(w + pbytes);
   f->result = (double *)(w + pbytes + 256);
   f->counter = (unsigned long long *)(w + pbytes + 256 + 16 * sizeof(double));
-  e = hipMemset(w, 0, ws);
+  // Both clears go on the library stream and are waited for: that stream is
+  // non-blocking, so a clear on the null stream would not be ordered before
+  // the initialiser kernel that follows, and could land on links it had
+  // already written.
+  e = hipMemsetAsync(w, 0, ws, ki::stream());
   // the links start zeroed, not as whatever the allocation held: the
   // reference has no field without an initialiser (GaugeField.hpp:33-62),
   // and here one of the set_* calls or an upload gives the start
-  if (e == hipSuccess) e = hipMemset(f->links, 0, f->link_bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(f->links, 0, f->link_bytes, ki::stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ki::stream());
   if (e == hipSuccess) e = hipHostMalloc((void **)&f->host_result, 16 * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) {
     (void)hipFree(w);
@@ -704,7 +709,8 @@ extern "C" int klft_plaquette(klft_field_t f, int normalize, double *out, double
   double sum = 0.0;
   if (!rc) rc = ki::read_results(f, 0, 1, &sum);
   if (!rc && per_site) {
-    hipError_t e = hipMemcpy(per_site, dps, sizeof(double) * f->g.V, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpyAsync(per_site, dps, sizeof(double) * f->g.V, hipMemcpyDeviceToHost, ki::stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(ki::stream());
     if (e != hipSuccess) rc = ki::hip_fail(e, "per-site copy");
   }
   if (dps) (void)hipFree(dps);

@@ -397,7 +397,10 @@ extern "C" int klft_hmc_create(klft_field_t f, const klft_hmc_params *p, klft_hm
   const size_t pbytes = (size_t)f->g.V * f->nd * ngen(f->nc) * rbytes;
   hipError_t e = hipMalloc(&h->P, pbytes);
   for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipMalloc(&h->Ubuf[i], f->link_bytes);
-  if (e == hipSuccess) e = hipMemset(h->P, 0, pbytes);
+  // cleared on the (non-blocking) library stream, where the kernels that
+  // read P run, and waited for
+  if (e == hipSuccess) e = hipMemsetAsync(h->P, 0, pbytes, ki::stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ki::stream());
   if (e != hipSuccess) {
     if (h->P) (void)hipFree(h->P);
     for (void *b : h->Ubuf)

@@ -73,6 +73,22 @@ def test_fp32_plaquette_within_1e6(field, storage):
     assert abs(klft.GaugePlaquette(g) - orc.plaquette(U, 4, 3, dims)) < 1e-6
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32_rows"])
+def test_new_fields_keep_their_start(storage):
+    """Fields created back to back (each over memory the previous one freed)
+    and started at once keep every link of their start: the creation's clear
+    of the links is ordered before the initialiser on the library stream.
+    Each link of an exp(i eps H) start is unitary, so no row may be zero."""
+    dims = (24, 24, 24, 24)
+    for rep in range(3):
+        g = klft.GaugeField(4, 3, dims, storage).set_random_algebra(0.5, 2024 + rep)
+        U = g.download()
+        g.close()
+        rows = np.sqrt((np.abs(U) ** 2).sum(axis=-1))
+        assert np.isfinite(U).all()
+        assert np.abs(rows - 1.0).max() < 1e-6, (rep, float(rows.min()))
+
+
 def test_lattice_beyond_65536_reduction_blocks():
     """48^3 x 40 (4.4 M sites, 69,120 link blocks): every reduction grid is
     larger than 2^16 blocks.  The field is an 8^4 oracle field repeated

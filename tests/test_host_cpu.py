@@ -52,6 +52,23 @@ def test_library_marks_launches_with_the_reference_labels():
         assert label in blob, label
 
 
+def test_no_null_stream_memory_operations():
+    """The library's kernels run on one non-blocking stream, which is not
+    ordered after work on the null stream: a hipMemset or hipMemcpy there can
+    still be running when a later kernel starts (a field's clear of its links
+    once landed after the initialiser had written them, leaving zero links
+    that the sweep turned into NaN).  Every memset and copy in the library is
+    the Async form on the library stream."""
+    bad = []
+    for p in sorted((ROOT / "klft_amd" / "csrc").rglob("*")):
+        if p.suffix not in (".hip", ".hpp", ".cpp", ".h"):
+            continue
+        src = re.sub(r"//[^\n]*", "", p.read_text())
+        for m in re.finditer(r"\bhip(Memset|Memcpy)(D32|D16|D8|2D|3D|Peer|ToSymbol|FromSymbol)?\s*\(", src):
+            bad.append(f"{p.relative_to(ROOT)}:{src.count(chr(10), 0, m.start()) + 1}: {m.group(0)}")
+    assert not bad, bad
+
+
 def test_python_binding_covers_the_header():
     import klft_amd
     klft_amd.load()
