@@ -177,44 +177,57 @@ __global__ __launch_bounds__(kB) void k_batch_reduce(const ChainDev *__restrict_
   if (threadIdx.x == 0) out[c] = v;
 }
 
-// A1 / A2 of every replica: one thread per replica visits the (0, nu)
-// plaquettes whose mu = 0 links touch the defect, in a fixed order
+// A1 / A2 of every replica: one wave per replica; lane l takes every 64th
+// of the (0, nu) plaquettes whose mu = 0 links touch the defect (nu = 1..3,
+// then the base sites in lexicographic order), and the lanes' sums are added
+// by a shuffle tree, so the result does not depend on scheduling.  A single
+// thread walking them one by one waited on each plaquette's loads in turn.
 template <typename R, int NC, bool COMP>
-__global__ void k_defect_sums(const ChainDev *__restrict__ chains, int n, Defect d, double *A) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_defect_sums(const ChainDev *__restrict__ chains, int n, Defect d,
+                                                    double *A) {
+  const int c = blockIdx.x;
   if (c >= n) return;
   const ChainDev ch = chains[c];
   const Geom &g = ch.g;
   Links<R, NC, COMP> f{reinterpret_cast<cx<R> *>(ch.links), g.V, 4};
-  double a1 = 0.0, a2 = 0.0;
+  // base sites y: y0 = 0, y_nu in [p_nu - 1, p_nu + Ld), the others in
+  // [p_k, p_k + Ld); when Ld equals L_nu, p_nu - 1 and p_nu + Ld - 1 are
+  // the same site, so the range along nu is cut to L_nu sites
+  int cnt[4] = {0, 0, 0, 0};
   for (int nu = 1; nu < 4; ++nu) {
-    // base sites y: y0 = 0, y_nu in [p_nu - 1, p_nu + Ld), the others in
-    // [p_k, p_k + Ld); when Ld equals L_nu, p_nu - 1 and p_nu + Ld - 1 are
-    // the same site, so the range along nu is cut to L_nu sites
+    const int enu = d.len + 1 < g.L[nu] ? d.len + 1 : g.L[nu];
+    cnt[nu] = enu * d.len * d.len;
+  }
+  const int total = cnt[1] + cnt[2] + cnt[3];
+  double a1 = 0.0, a2 = 0.0;
+  for (int t = (int)threadIdx.x; t < total; t += 64) {
+    int nu = 1, r = t;
+    while (r >= cnt[nu]) r -= cnt[nu++];
     int ext[4] = {1, d.len, d.len, d.len};
     ext[nu] = d.len + 1 < g.L[nu] ? d.len + 1 : g.L[nu];
-    for (int i1 = 0; i1 < ext[1]; ++i1)
-      for (int i2 = 0; i2 < ext[2]; ++i2)
-        for (int i3 = 0; i3 < ext[3]; ++i3) {
-          int y[4] = {0, d.pos[0] + i1, d.pos[1] + i2, d.pos[2] + i3};
-          y[nu] -= 1;
-          for (int k = 1; k < 4; ++k) y[k] = ((y[k] % g.L[k]) + g.L[k]) % g.L[k];
-          int yn[4], y0[4];
-          shifted<4>(g, y, nu, 1, -1, 0, yn);
-          shifted<4>(g, y, 0, 1, -1, 0, y0);
-          const int k = (int)on_defect<4>(g, d, y, 0) + (int)on_defect<4>(g, d, yn, 0);
-          if (k == 0) continue;
-          const Mat<R, NC> lmu = mul(f.load(dev_index<4>(g, y), 0), f.load(dev_index<4>(g, y0), nu));
-          const Mat<R, NC> lnu = mul(f.load(dev_index<4>(g, y), nu), f.load(dev_index<4>(g, yn), 0));
-          const double p = (double)retr_mul_ad(lmu, lnu);
-          if (k == 1)
-            a1 += p;
-          else
-            a2 += p;
-        }
+    const int i3 = r % ext[3], i2 = (r / ext[3]) % ext[2], i1 = r / (ext[3] * ext[2]);
+    int y[4] = {0, d.pos[0] + i1, d.pos[1] + i2, d.pos[2] + i3};
+    y[nu] -= 1;
+    for (int k = 1; k < 4; ++k) y[k] = ((y[k] % g.L[k]) + g.L[k]) % g.L[k];
+    int yn[4], y0[4];
+    shifted<4>(g, y, nu, 1, -1, 0, yn);
+    shifted<4>(g, y, 0, 1, -1, 0, y0);
+    const int k = (int)on_defect<4>(g, d, y, 0) + (int)on_defect<4>(g, d, yn, 0);
+    if (k == 0) continue;
+    const Mat<R, NC> lmu = mul(f.load(dev_index<4>(g, y), 0), f.load(dev_index<4>(g, y0), nu));
+    const Mat<R, NC> lnu = mul(f.load(dev_index<4>(g, y), nu), f.load(dev_index<4>(g, yn), 0));
+    const double p = (double)retr_mul_ad(lmu, lnu);
+    if (k == 1)
+      a1 += p;
+    else
+      a2 += p;
   }
-  A[2 * c] = a1;
-  A[2 * c + 1] = a2;
+  a1 = wave_sum(a1);
+  a2 = wave_sum(a2);
+  if (threadIdx.x == 0) {
+    A[2 * c] = a1;
+    A[2 * c + 1] = a2;
+  }
 }
 
 // sequential nearest-neighbour swap proposals of the coupling labels
@@ -239,10 +252,10 @@ __global__ void k_swap(int n, int nc, double beta, const double *__restrict__ cv
   for (int k = 0; k < n; ++k) cval[conf_of[k]] = cvals[k];
 }
 
-// A1 / A2 of every replica into A (one thread per replica)
+// A1 / A2 of every replica into A (one wave per replica)
 template <typename R, bool COMP>
 void launch_defect_sums(int nc, const ChainDev *chains, int n, const Defect &d, double *A) {
-  const dim3 grid((unsigned)((n + 63) / 64)), block(64);
+  const dim3 grid((unsigned)n), block(64);
   switch (nc) {
     case 1: hipLaunchKernelGGL((k_defect_sums<R, 1, false>), grid, block, 0, ki::stream(), chains, n, d, A); break;
     case 2: hipLaunchKernelGGL((k_defect_sums<R, 2, COMP>), grid, block, 0, ki::stream(), chains, n, d, A); break;
