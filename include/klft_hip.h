@@ -151,6 +151,16 @@ int klft_metropolis_schedule(klft_field_t f, int32_t *visit);
  * launches every pass on its own (Metropolis.hpp:139-162); for callers and
  * tests that check the schedule.  Host only: no GPU call. */
 int klft_sweep_launches(int nd, int32_t *groups, int *nlaunches);
+/* the launch order of klft_metropolis_run over nsweeps sweeps: launch i
+ * runs pass launches[4i] of sweep launches[4i+1] (relative to the run's
+ * first) and pass launches[4i+2] of sweep launches[4i+3] (-1, -1: none).
+ * Independent passes of consecutive sweeps share launches too (13|0', 14|1',
+ * 15|2' in 4D: 8 launches per sweep over a long run).  *nlaunches receives
+ * the count; launches holds 4 * capacity ints (an error when too small).
+ * No counterpart in the reference (Metropolis.hpp:139-162 launches pass by
+ * pass, sweep by sweep); for callers and tests that check the schedule.
+ * Host only: no GPU call. */
+int klft_run_launches(int nd, int nsweeps, int32_t *launches, int capacity, int *nlaunches);
 
 /* ---- HMC (HMC.hpp:48-71, Integrator.hpp:26-53, GaugeMonomial.hpp:19-37,
  *      HamiltonianField.hpp:23-60) ------------------------------------------------ */

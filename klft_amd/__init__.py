@@ -115,6 +115,7 @@ _SIGS = {
     "klft_metropolis_run": (C.c_int, [_p, C.POINTER(MetropolisParamsC), C.c_uint64, C.c_int, _dp]),
     "klft_metropolis_schedule": (C.c_int, [_p, _i32p]),
     "klft_sweep_launches": (C.c_int, [C.c_int, _i32p, C.POINTER(C.c_int)]),
+    "klft_run_launches": (C.c_int, [C.c_int, C.c_int, _i32p, C.c_int, C.POINTER(C.c_int)]),
     "klft_field_set_proposal": (C.c_int, [_p, C.c_int]),
     "klft_hmc_create": (C.c_int, [_p, C.POINTER(HmcParamsC), C.POINTER(_p)]),
     "klft_hmc_destroy": (C.c_int, [_p]),
@@ -358,6 +359,18 @@ def sweep_launches(nd: int):
     n = C.c_int()
     _check(lib.klft_sweep_launches(nd, g.ctypes.data_as(_i32p), C.byref(n)))
     return [tuple(int(p) for p in g[2 * i:2 * i + 2] if p >= 0) for i in range(n.value)]
+
+
+def run_launches(nd: int, nsweeps: int):
+    """The launches of metropolis_run over nsweeps sweeps: a list of
+    ((pass, sweep), ...) tuples in launch order, sweeps relative to the
+    run's first (independent passes of consecutive sweeps share launches)."""
+    cap = (1 << nd) * max(nsweeps, 1)
+    g = np.empty(4 * cap, dtype=np.int32)
+    n = C.c_int()
+    _check(lib.klft_run_launches(nd, nsweeps, g.ctypes.data_as(_i32p), cap, C.byref(n)))
+    return [tuple((int(g[4 * i + 2 * k]), int(g[4 * i + 2 * k + 1])) for k in range(2) if g[4 * i + 2 * k] >= 0)
+            for i in range(n.value)]
 
 
 class HMC:

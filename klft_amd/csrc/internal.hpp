@@ -9,6 +9,7 @@
 #include <functional>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include <rocprofiler-sdk-roctx/roctx.h>
 
@@ -138,6 +139,48 @@ inline PassGroups pass_groups(int nd) {
     }
   }
   return gr;
+}
+
+// The same rule across the sweeps of a run: pass p of sweep s conflicts with
+// every earlier pass (of this sweep or an earlier one) whose sublattice is at
+// most two parities from p's, its own sublattice included.  Passes are taken
+// in program order (sweep-major, 0 .. 2^nd - 1 within a sweep) and each goes
+// into the first launch after the last conflicting pass that still has room
+// for it, at most two passes to a launch; two conflicting passes therefore
+// keep their order, and a pass may move ahead of earlier passes it does not
+// touch.  One sweep gives pass_groups' 11 launches in 4D; over many sweeps
+// the first three passes of each sweep pair with the last three of the one
+// before (13|0', 14|1', 15|2', all four parities apart), 8 launches per
+// sweep, every one of them a pair.  sweep[] is relative to the run's first.
+struct RunLaunch {
+  int pass[2];
+  int sweep[2];
+};
+inline std::vector<RunLaunch> run_schedule(int nd, int nsweeps) {
+  const int np = 1 << nd;
+  std::vector<RunLaunch> out;
+  int last[16];  // launch of the latest scheduled pass of each sublattice
+  for (int q = 0; q < np; ++q) last[q] = -1;
+  std::vector<int> fill;
+  for (int s = 0; s < nsweeps; ++s) {
+    for (int p = 0; p < np; ++p) {
+      int lo = 0;
+      for (int q = 0; q < np; ++q)
+        if (__builtin_popcount(p ^ q) <= 2 && last[q] + 1 > lo) lo = last[q] + 1;
+      int k = lo;
+      while (k < (int)out.size() && fill[k] >= 2) ++k;
+      if (k == (int)out.size()) {
+        out.push_back(RunLaunch{{p, -1}, {s, 0}});
+        fill.push_back(1);
+      } else {
+        out[k].pass[fill[k]] = p;
+        out[k].sweep[fill[k]] = s;
+        ++fill[k];
+      }
+      last[p] = k;
+    }
+  }
+  return out;
 }
 
 // compute units of the current device (256 on an MI355X), queried once

@@ -403,19 +403,25 @@ __device__ __forceinline__ void ws_count(unsigned long long n, unsigned long lon
   }
 }
 
-// pass2 >= 0: the launch runs the independent passes `pass` and `pass2`
-// (pass_groups), the odd logical blocks taking pass2, so that the blocks of
+// pass2 >= 0: the launch runs the independent passes `pass` of sweep
+// `sweep` and `pass2` of sweep `sweep2` (pass_groups, run_schedule), the odd
+// logical blocks taking pass2 and counting into nacc2, so that the blocks of
 // one XCD hold the same chunk of both sublattices
 template <typename R, int NC, bool COMP, int ND, bool ILV, bool SYM = false>
 __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
     Links<R, NC, COMP> f, Geom g, int pass, int pass2, int nhits, R beta_nc, R delta, uint64_t seed,
-    uint32_t sweep, uint32_t chain, unsigned long long *nacc, int *visit) {
+    uint32_t sweep, uint32_t sweep2, uint32_t chain, unsigned long long *nacc, unsigned long long *nacc2,
+    int *visit) {
   f.nd = ND;
   __shared__ cx<R> stage[4 * 4 * NC * NC * 64];
   __shared__ unsigned long long red[kWsBlock / 64];
   unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   if (pass2 >= 0) {
-    if (lb & 1u) pass = pass2;
+    if (lb & 1u) {
+      pass = pass2;
+      sweep = sweep2;
+      nacc = nacc2;
+    }
     lb >>= 1;
   }
   const long long t = (long long)lb * kWsSites + (long long)((threadIdx.x >> 6) & 3) * 64 + (threadIdx.x & 63);
@@ -435,8 +441,8 @@ __global__ __launch_bounds__(kWsBlock) void k_metropolis_ws(
 // 16^4 - 48^4 and were removed.
 template <typename RR, bool CC>
 int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1,
-                        const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
-                        unsigned long long *counter, int *visit) {
+                        const klft_metropolis_params *p, uint64_t sweep, int nsweeps, uint32_t chain,
+                        unsigned long long *counter, int counter_stride, int *visit) {
   return dispatch_typed<RR, CC>(f->nc, f->nd, [&](auto rt, auto nct, auto ct, auto ndt) -> int {
     KLFT_UNPACK(rt, nct, ct, ndt);
     Links<R, NC, COMP> l;
@@ -449,31 +455,37 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
       case 0: case 4: case 7: case 11: case 12: case 13: case 14: case 15: break;
       default: return set_error(KLFT_E_ARG, "sweep variant %d is not built", variant);
     }
+    // acceptance counter of sweep i of the run (relative to `sweep`)
+    auto cnt = [&](int i) { return counter + (long long)i * counter_stride; };
     const unsigned nws = blocks_for(f->g.Vh, kWsSites);
-    // launch the site-wave kernel on one pass, or on two independent ones
-    auto ws = [&](auto kern, int pa, int pb) {
+    // launch the site-wave kernel on one pass (pb < 0), or on two
+    // independent ones, each with its own sweep and counter
+    auto ws = [&](auto kern, int pa, int sa, int pb, int sb) {
       hipLaunchKernelGGL(kern, dim3(pb >= 0 ? 2 * nws : nws), dim3(kWsBlock), 0, stream(), l, f->g, pa, pb,
-                         p->nhits, beta_nc, (R)p->delta, p->seed, (uint32_t)sweep, chain, counter, visit);
+                         p->nhits, beta_nc, (R)p->delta, p->seed, (uint32_t)(sweep + sa), (uint32_t)(sweep + sb),
+                         chain, cnt(sa), cnt(sb), visit);
     };
-    auto launch_ws = [&](int pa, int pb) -> int {
+    auto launch_ws = [&](int pa, int sa, int pb, int sb) -> int {
       switch (variant) {
-        case 12: ws(k_metropolis_ws<R, NC, COMP, ND, false>, pa, pb); break;
-        case 13: ws(k_metropolis_ws<R, NC, COMP, ND, true>, pa, pb); break;
+        case 12: ws(k_metropolis_ws<R, NC, COMP, ND, false>, pa, sa, pb, sb); break;
+        case 13: ws(k_metropolis_ws<R, NC, COMP, ND, true>, pa, sa, pb, sb); break;
         default:
           // the symmetric SU(3) proposal: site waves, terms as in 12 / 13
           if constexpr (NC == 3) {
             if (variant == 14)
-              ws(k_metropolis_ws<R, NC, COMP, ND, false, true>, pa, pb);
+              ws(k_metropolis_ws<R, NC, COMP, ND, false, true>, pa, sa, pb, sb);
             else
-              ws(k_metropolis_ws<R, NC, COMP, ND, true, true>, pa, pb);
+              ws(k_metropolis_ws<R, NC, COMP, ND, true, true>, pa, sa, pb, sb);
           } else {
             return set_error(KLFT_E_ARG, "the symmetric proposal variants are SU(3) only");
           }
       }
       return KLFT_OK;
     };
-    // a whole sweep runs its independent passes in pairs when both passes'
-    // blocks fit on the GPU at once (KLFT_VARIANT=metropolis_pairs=0: never)
+    // whole sweeps run their independent passes in pairs when both passes'
+    // blocks fit on the GPU at once (KLFT_VARIANT=metropolis_pairs=0: never);
+    // a run of several sweeps also pairs passes of consecutive sweeps
+    // (run_schedule)
     bool pair = false;
     if (variant >= 12 && p0 == 0 && p1 == (1 << ND) && forced_variant("metropolis_pairs") != 0) {
       const void *kern = nullptr;
@@ -486,36 +498,38 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
       pair = 2LL * nws <= (long long)cu_count() * blocks_per_cu(kern, kWsBlock);
     }
     if (pair) {
-      const PassGroups gr = pass_groups(ND);
-      for (int i = 0; i < gr.n; ++i) {
-        const int pa = gr.p[i][0], pb = gr.p[i][1];
+      const std::vector<RunLaunch> sched = run_schedule(ND, nsweeps);
+      for (const RunLaunch &L : sched) {
+        const int pa = L.pass[0], pb = L.pass[1];
         char label[56];
         if (pb >= 0)
           snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d_%d", pa, pb);
         else
           snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pa);  // Metropolis.hpp:161
         ProfScope prof(pb >= 0 ? "metropolis_pass_pair" : "metropolis_pass", label);
-        const int rc = launch_ws(pa, pb);
+        const int rc = launch_ws(pa, L.sweep[0], pb, pb >= 0 ? L.sweep[1] : L.sweep[0]);
         if (rc) return rc;
       }
     } else {
-      for (int pass = p0; pass < p1; ++pass) {
-        char label[48];
-        snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pass);  // Metropolis.hpp:161
-        ProfScope prof("metropolis_pass", label);
-        if (variant >= 12) {
-          const int rc = launch_ws(pass, -1);
-          if (rc) return rc;
-        } else {
-          auto one_lane = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(kMetroBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc,
-                               (R)p->delta, p->seed, (uint32_t)sweep, chain, counter, visit);
-          };
-          switch (variant) {
-            case 4: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2>); break;
-            case 7: one_lane(k_metropolis_pipe<R, NC, COMP, ND>); break;
-            case 11: one_lane(k_metropolis_own<R, NC, COMP, ND>); break;
-            default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1>); break;
+      for (int i = 0; i < nsweeps; ++i) {
+        for (int pass = p0; pass < p1; ++pass) {
+          char label[48];
+          snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pass);  // Metropolis.hpp:161
+          ProfScope prof("metropolis_pass", label);
+          if (variant >= 12) {
+            const int rc = launch_ws(pass, i, -1, i);
+            if (rc) return rc;
+          } else {
+            auto one_lane = [&](auto kern) {
+              hipLaunchKernelGGL(kern, dim3(nb), dim3(kMetroBlock), 0, stream(), l, f->g, pass, p->nhits, beta_nc,
+                                 (R)p->delta, p->seed, (uint32_t)(sweep + i), chain, cnt(i), visit);
+            };
+            switch (variant) {
+              case 4: one_lane(k_metropolis_pass<R, NC, COMP, ND, true, 2>); break;
+              case 7: one_lane(k_metropolis_pipe<R, NC, COMP, ND>); break;
+              case 11: one_lane(k_metropolis_own<R, NC, COMP, ND>); break;
+              default: one_lane(k_metropolis_pass<R, NC, COMP, ND, false, 1>); break;
+            }
           }
         }
       }

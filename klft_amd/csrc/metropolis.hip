@@ -26,12 +26,14 @@ constexpr int kMetroVariants = 14;
 
 namespace klft_internal {
 int launch_passes_f64(klft_field_t f, void *links, int variant, int p0, int p1, const klft_metropolis_params *p,
-                      uint64_t sweep, uint32_t chain, unsigned long long *counter, int *visit);
+                      uint64_t sweep, int nsweeps, uint32_t chain, unsigned long long *counter,
+                      int counter_stride, int *visit);
 int launch_passes_f32(klft_field_t f, void *links, int variant, int p0, int p1, const klft_metropolis_params *p,
-                      uint64_t sweep, uint32_t chain, unsigned long long *counter, int *visit);
+                      uint64_t sweep, int nsweeps, uint32_t chain, unsigned long long *counter,
+                      int counter_stride, int *visit);
 int launch_passes_f32rows(klft_field_t f, void *links, int variant, int p0, int p1,
-                          const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
-                          unsigned long long *counter, int *visit);
+                          const klft_metropolis_params *p, uint64_t sweep, int nsweeps, uint32_t chain,
+                          unsigned long long *counter, int counter_stride, int *visit);
 }  // namespace klft_internal
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -41,21 +43,26 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 
 namespace klft_internal {
 
-// launch passes [p0, p1) of one sweep on the link array `links`; the kernels
-// are instantiated per storage in metropolis_{f64,f32,f32rows}.hip (the
-// variant table is in metro_kernels.hpp).  U(1) has no compressed form, so
-// its two-row storage runs the FP32 kernels.
+// launch passes [p0, p1) of sweeps sweep .. sweep + nsweeps - 1 on the link
+// array `links`, sweep i counting its accepted hits into
+// counter[i * counter_stride]; the kernels are instantiated per storage in
+// metropolis_{f64,f32,f32rows}.hip (the variant table is in
+// metro_kernels.hpp).  U(1) has no compressed form, so its two-row storage
+// runs the FP32 kernels.
 static int launch_passes(klft_field_t f, void *links, int variant, int p0, int p1,
-                         const klft_metropolis_params *p, uint64_t sweep, uint32_t chain,
-                         unsigned long long *counter, int *visit) {
+                         const klft_metropolis_params *p, uint64_t sweep, int nsweeps, uint32_t chain,
+                         unsigned long long *counter, int counter_stride, int *visit) {
   switch (f->storage) {
     case KLFT_STORAGE_F64:
-      return launch_passes_f64(f, links, variant, p0, p1, p, sweep, chain, counter, visit);
+      return launch_passes_f64(f, links, variant, p0, p1, p, sweep, nsweeps, chain, counter, counter_stride, visit);
     case KLFT_STORAGE_F32:
-      return launch_passes_f32(f, links, variant, p0, p1, p, sweep, chain, counter, visit);
+      return launch_passes_f32(f, links, variant, p0, p1, p, sweep, nsweeps, chain, counter, counter_stride, visit);
     case KLFT_STORAGE_F32_ROWS:
-      if (f->nc == 1) return launch_passes_f32(f, links, variant, p0, p1, p, sweep, chain, counter, visit);
-      return launch_passes_f32rows(f, links, variant, p0, p1, p, sweep, chain, counter, visit);
+      if (f->nc == 1)
+        return launch_passes_f32(f, links, variant, p0, p1, p, sweep, nsweeps, chain, counter, counter_stride,
+                                 visit);
+      return launch_passes_f32rows(f, links, variant, p0, p1, p, sweep, nsweeps, chain, counter, counter_stride,
+                                   visit);
   }
   return set_error(KLFT_E_ARG, "storage=%d unsupported", f->storage);
 }
@@ -96,7 +103,7 @@ int metropolis_variant(klft_field_t f, const klft_metropolis_params *p, bool tim
       KLFT_HIP(hipMalloc(&scratch, f->link_bytes));
       KLFT_HIP(hipMemcpyAsync(scratch, f->links, f->link_bytes, hipMemcpyDeviceToDevice, stream()));
     }
-    return launch_passes(f, scratch, var, 0, 1 << f->nd, p, 0, 0, f->counter + 7, nullptr);
+    return launch_passes(f, scratch, var, 0, 1 << f->nd, p, 0, 1, 0, f->counter + 7, 0, nullptr);
   });
   if (scratch) {
     (void)hipStreamSynchronize(stream());
@@ -111,7 +118,17 @@ int enqueue_sweep(klft_field_t f, const klft_metropolis_params *p, uint64_t swee
   // the schedule dump (visit != nullptr, nHits = 0) is never timed: its
   // staple-only passes are not the workload a tuned variant is chosen for
   const int v = metropolis_variant(f, p, visit == nullptr);
-  return launch_passes(f, f->links, v, 0, 1 << f->nd, p, sweep, chain, counter, visit);
+  return launch_passes(f, f->links, v, 0, 1 << f->nd, p, sweep, 1, chain, counter, 0, visit);
+}
+
+// enqueue sweeps sweep0 .. sweep0 + n - 1 (no synchronisation), passes of
+// consecutive sweeps sharing launches where they are independent; sweep i
+// counts into counter[i * counter_stride]
+static int enqueue_run(klft_field_t f, const klft_metropolis_params *p, uint64_t sweep0, int n,
+                       unsigned long long *counter, int counter_stride) {
+  if (n <= 0) return KLFT_OK;
+  const int v = metropolis_variant(f, p, true);
+  return launch_passes(f, f->links, v, 0, 1 << f->nd, p, sweep0, n, 0, counter, counter_stride, nullptr);
 }
 
 }  // namespace klft_internal
@@ -173,7 +190,7 @@ extern "C" int klft_metropolis_run(klft_field_t f, const klft_metropolis_params 
   } else {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ki::stream(), f->counter, 1);
   }
-  for (int i = 0; i < n && !rc; ++i) rc = ki::enqueue_sweep(f, p, sweep0 + i, 0, cnt ? cnt + i : f->counter, nullptr);
+  rc = ki::enqueue_run(f, p, sweep0, n, cnt ? cnt : f->counter, cnt ? 1 : 0);
   if (!rc && cnt) {
     std::vector<unsigned long long> h(n);
     hipError_t e = hipMemcpyAsync(h.data(), cnt, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost,
@@ -195,6 +212,23 @@ extern "C" int klft_sweep_launches(int nd, int32_t *groups, int *nlaunches) {
     groups[2 * i + 1] = gr.p[i][1];
   }
   *nlaunches = gr.n;
+  return KLFT_OK;
+}
+
+extern "C" int klft_run_launches(int nd, int nsweeps, int32_t *launches, int capacity, int *nlaunches) {
+  if (nd < 2 || nd > 4) return ki::set_error(KLFT_E_ARG, "nd=%d unsupported (2..4)", nd);
+  if (nsweeps < 0) return ki::set_error(KLFT_E_ARG, "negative sweep count");
+  if (!nlaunches || (capacity > 0 && !launches)) return ki::set_error(KLFT_E_ARG, "null argument");
+  const std::vector<ki::RunLaunch> sched = ki::run_schedule(nd, nsweeps);
+  *nlaunches = (int)sched.size();
+  if ((int)sched.size() > capacity)
+    return ki::set_error(KLFT_E_ARG, "%d launches do not fit a capacity of %d", (int)sched.size(), capacity);
+  for (size_t i = 0; i < sched.size(); ++i) {
+    launches[4 * i] = sched[i].pass[0];
+    launches[4 * i + 1] = sched[i].sweep[0];
+    launches[4 * i + 2] = sched[i].pass[1];
+    launches[4 * i + 3] = sched[i].pass[1] >= 0 ? sched[i].sweep[1] : -1;
+  }
   return KLFT_OK;
 }
 

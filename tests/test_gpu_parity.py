@@ -346,6 +346,34 @@ def test_paired_passes_equal_sequential_sweep(monkeypatch, storage, dims, nc, va
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("storage,dims,nc,variant", [
+    ("f64", (12, 12, 8, 10), 3, 13), ("f32", (16, 16, 16, 16), 3, 13), ("f32_rows", (16, 16, 16, 16), 3, 12),
+    ("f32", (32, 32, 32, 32), 3, 12), ("f64", (10, 8, 12), 3, 12), ("f64", (8, 8, 8, 8), 2, 13),
+    ("f32", (6, 6, 6, 6), 1, 12), ("f64", (8, 8, 8, 8), 3, 15)])
+def test_run_pairing_across_sweeps_equals_sequential_sweeps(monkeypatch, storage, dims, nc, variant):
+    """metropolis_run over several sweeps also runs independent passes of
+    consecutive sweeps in one launch (4D: 13|0', 14|1', 15|2'); the links and
+    every sweep's acceptance rate equal those of sweeps run one by one with
+    no pairing, bit for bit (variant 15: the symmetric SU(3) proposal)."""
+    nd = len(dims)
+    beta = {1: 2.0, 2: 2.3, 3: 6.0}[nc]
+    p = klft.MetropolisParams(nHits=10, beta=beta, delta=0.1, seed=4242)
+    out = []
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("KLFT_VARIANT", f"metropolis={variant},metropolis_pairs={pairs}")
+        g = klft.GaugeField(nd, nc, dims, storage).set_random_algebra(0.5, 2024)
+        if variant == 15:
+            g.set_proposal("symmetric")
+        if pairs == "0":
+            rates = [klft.sweep_Metropolis(g, p, s) for s in range(3, 8)]
+        else:
+            rates = list(klft.metropolis_run(g, p, 3, 5))
+        out.append((rates, g.download()))
+        g.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32", "f32_rows"])
 def test_new_field_links_start_zeroed(storage):
     """A field's links read back as zeros before any initialiser runs."""

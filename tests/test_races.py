@@ -135,3 +135,47 @@ def test_pairing_dependent_passes_would_race(pair):
     merged = visit.copy()
     merged[visit == pair[1]] = pair[0]
     assert pass_conflicts(nd, dims, merged) > 0
+
+
+@pytest.mark.parametrize("nd,nsweeps,nlaunch", [(2, 5, 20), (3, 5, 31), (4, 1, 11), (4, 5, 43), (4, 50, 403)])
+def test_run_launch_order_keeps_every_dependency(nd, nsweeps, nlaunch):
+    """The library's launch order for a run of sweeps (klft_run_launches, the
+    schedule metropolis_run follows): every (sweep, pass) once, at most two
+    to a launch, the two of a launch three or more parities apart, and of two
+    passes that touch the same links (parities at most two apart, the same
+    sublattice of another sweep included) the earlier one in the reference's
+    order (sweep by sweep, passes 0..2^nd-1, Metropolis.hpp:139-162) runs in
+    an earlier launch.  One sweep is the single-sweep order; over many sweeps
+    4D takes 8 launches per sweep instead of 11."""
+    import klft_amd as klft
+    launches = klft.run_launches(nd, nsweeps)
+    assert len(launches) == nlaunch
+    if nsweeps == 1:
+        assert [tuple(p for p, _ in launch) for launch in launches] == klft.sweep_launches(nd)
+    flat = [sp for launch in launches for sp in launch]
+    assert sorted(flat) == sorted((p, s) for s in range(nsweeps) for p in range(1 << nd))
+    at = {sp: i for i, launch in enumerate(launches) for sp in launch}
+    for launch in launches:
+        assert len(launch) <= 2
+        if len(launch) == 2:
+            assert bin(launch[0][0] ^ launch[1][0]).count("1") >= 3
+    order = [(p, s) for s in range(nsweeps) for p in range(1 << nd)]
+    for i, a in enumerate(order):
+        for b in order[i + 1:]:
+            if bin(a[0] ^ b[0]).count("1") <= 2:
+                assert at[a] < at[b], (a, b)
+
+
+@pytest.mark.parametrize("nd,dims", [(3, (4, 6, 8)), (4, (4, 4, 6, 4)), (4, (2, 4, 2, 6))])
+def test_run_paired_passes_are_race_free(nd, dims):
+    """Every pair of passes that a run of sweeps puts in one launch (passes of
+    consecutive sweeps included), emulated as one pass over both passes'
+    sites: no link read on behalf of one site is written by another."""
+    import klft_amd as klft
+    visit = schedule(nd, dims)
+    pairs = {tuple(sorted(p for p, _ in launch)) for launch in klft.run_launches(nd, 4) if len(launch) == 2}
+    assert pairs
+    for a, b in sorted(pairs):
+        merged = visit.copy()
+        merged[visit == b] = a
+        assert pass_conflicts(nd, dims, merged) == 0, (a, b)
