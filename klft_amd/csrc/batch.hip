@@ -101,23 +101,25 @@ __global__ __launch_bounds__(kB) void k_batch_pass(const ChainDev *__restrict__ 
 // with their passes paired)
 template <typename R, bool COMP, bool DEFECT, bool ILV = false>
 __global__ __launch_bounds__(klft_metro::kWsBlock, (sizeof(R) == 4 && !ILV) ? 4 : 2) void k_batch_ws(
-    const ChainDev *__restrict__ chains, const int *__restrict__ block_chain, int pass, int pass2, int nhits,
-    R delta, uint64_t seed, const uint32_t *__restrict__ d_sweep, unsigned long long *counters, Defect d,
+    const ChainDev *__restrict__ chains, const int *__restrict__ block_chain, int pass, int pass2, int ds, int ds2,
+    int nhits, R delta, uint64_t seed, const uint32_t *__restrict__ d_sweep, unsigned long long *counters, Defect d,
     const double *__restrict__ cval) {
   __shared__ cx<R> stage[4 * 4 * 9 * 64];
   __shared__ unsigned long long red[klft_metro::kWsBlock / 64];
-  // pass2 >= 0: two independent passes in one launch (pass_groups), the
-  // second half of the grid taking pass2
+  // pass2 >= 0: two independent passes in one launch (pass_groups,
+  // run_schedule), the second half of the grid taking pass2; the passes
+  // belong to sweeps *d_sweep + ds and *d_sweep + ds2
   unsigned int bid = blockIdx.x;
   if (pass2 >= 0 && bid >= gridDim.x / 2) {
     pass = pass2;
+    ds = ds2;
     bid -= gridDim.x / 2;
   }
   const int c = block_chain[bid];
   const ChainDev ch = chains[c];
   const long long t = (long long)((long long)bid - ch.first_block) * kB + (long long)((threadIdx.x >> 6) & 3) * 64 +
                       (threadIdx.x & 63);
-  const uint32_t sweep = *d_sweep;
+  const uint32_t sweep = *d_sweep + (uint32_t)ds;
   const R cc = DEFECT ? (R)cval[c] : R(1);
   unsigned long long n = 0;
   switch (ch.nc) {
@@ -285,6 +287,11 @@ struct klft_batch_s {
   // graph of one sweep / step
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // graphs of a run of sweeps whose passes pair across sweeps: head, window,
+  // tail (RunParts); run_pairs: whether the batch runs that way
+  hipGraph_t rgraph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t rexec[3] = {nullptr, nullptr, nullptr};
+  bool run_pairs = false;
   int g_nhits = -1;
   double g_delta = 0.0;
   uint64_t g_seed = 0;
@@ -304,9 +311,21 @@ int check_batch(klft_batch_t b) {
   return KLFT_OK;
 }
 
-void free_batch(klft_batch_t b) {
+void drop_graphs(klft_batch_t b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (int i = 0; i < 3; ++i) {
+    if (b->rexec[i]) (void)hipGraphExecDestroy(b->rexec[i]);
+    if (b->rgraph[i]) (void)hipGraphDestroy(b->rgraph[i]);
+    b->rexec[i] = nullptr;
+    b->rgraph[i] = nullptr;
+  }
+}
+
+void free_batch(klft_batch_t b) {
+  drop_graphs(b);
   for (auto f : b->fields) klft_field_destroy(f);
   void *ptrs[] = {b->d_chains, b->d_block_chain, b->d_pblock_chain, b->d_partials, b->d_plaq, b->d_counters,
                   b->d_sweep, b->d_cvals, b->d_cval, b->d_A, b->d_conf_of, b->d_swaps};
@@ -380,9 +399,61 @@ static void prime_occupancy(klft_batch_t b) {
   }
 }
 
-// enqueue one sweep (and, for replicas, the swap pass); no host sync
-int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
-  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, ki::stream(), b->d_counters, b->n);
+// One launch of the site-wave kernel: pass pa of sweep *d_sweep + da, and
+// pass pb of sweep *d_sweep + db (pb < 0: none).
+struct BLaunch {
+  int pa, da, pb, db;
+};
+// A run of n >= 2 sweeps of chains, with passes of consecutive sweeps
+// sharing launches (run_schedule): head = sweep 0's launches before its
+// first pair with sweep 1 (4D: {0} {1} {2} {3,4} {5,8} {6,9} {7,10}
+// {11,12}); window = the launches from there up to the first that holds a
+// pass of sweep 2 ({13,0'} {14,1'} {15,2'} {3',4'} ... {11',12'}), offsets
+// relative to the device sweep index, which the window then advances by
+// one; tail = the launches after the last window ({13} {14} {15}), then an
+// advance.  The run is head, n - 1 windows, tail.  ok is false if the
+// schedule does not repeat with period one sweep (it does for every nd).
+struct RunParts {
+  std::vector<BLaunch> part[3];
+  bool ok = false;
+};
+const RunParts &run_parts() {
+  static const RunParts rp = [] {
+    RunParts r;
+    const std::vector<ki::RunLaunch> S = ki::run_schedule(4, 3);
+    auto top = [](const ki::RunLaunch &L) {
+      return L.pass[1] >= 0 && L.sweep[1] > L.sweep[0] ? L.sweep[1] : L.sweep[0];
+    };
+    auto conv = [](const ki::RunLaunch &L, int base) {
+      return BLaunch{L.pass[0], L.sweep[0] - base, L.pass[1], L.pass[1] >= 0 ? L.sweep[1] - base : 0};
+    };
+    size_t k1 = 0, k2;
+    while (k1 < S.size() && top(S[k1]) < 1) ++k1;
+    for (k2 = k1; k2 < S.size() && top(S[k2]) < 2; ++k2) {
+    }
+    const size_t w = k2 - k1;
+    if (w == 0 || k2 + w > S.size()) return r;
+    for (size_t i = 0; i < k1; ++i) r.part[0].push_back(conv(S[i], 0));
+    for (size_t i = k1; i < k2; ++i) r.part[1].push_back(conv(S[i], 0));
+    for (size_t i = 0; i < w; ++i) {
+      const BLaunch a = conv(S[k2 + i], 1), c = r.part[1][i];
+      if (a.pa != c.pa || a.da != c.da || a.pb != c.pb || a.db != c.db) return r;
+    }
+    for (size_t i = k2 + w; i < S.size(); ++i) r.part[2].push_back(conv(S[i], 2));
+    r.ok = true;
+    return r;
+  }();
+  return rp;
+}
+
+// part 0: enqueue one sweep (and, for replicas, the swap pass), then advance
+// the sweep index; parts 1-3: the head, window and tail of a run whose
+// passes pair across sweeps (run_parts; chains only, and only when the
+// batch pairs passes).  No host sync.  *paired (optional) receives whether
+// independent passes share launches.
+int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed, int part = 0, bool *paired = nullptr) {
+  if (part <= 1)
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, ki::stream(), b->d_counters, b->n);
   auto passes = [&](auto rt, auto ct) -> int {
     using R = typename decltype(rt)::type;
     constexpr bool COMP = decltype(ct)::value != 0;
@@ -395,10 +466,10 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
     const int fv = ki::forced_variant("batch");
     const int v = fv >= 0 && fv <= 2 ? fv : ((sizeof(R) == 8 || 2 * b->nblocks <= ki::cu_count()) ? 2 : 1);
     const bool ws = v != 0;
-    auto site_waves = [&](auto kern, int pa, int pb) {
+    auto site_waves = [&](auto kern, int pa, int pb, int da = 0, int db = 0) {
       hipLaunchKernelGGL(kern, dim3(pb >= 0 ? 2 * b->nblocks : b->nblocks), dim3(klft_metro::kWsBlock), 0,
-                         ki::stream(), b->d_chains, b->d_block_chain, pa, pb, nhits, (R)delta, seed, b->d_sweep,
-                         b->d_counters, b->defect, b->d_cval);
+                         ki::stream(), b->d_chains, b->d_block_chain, pa, pb, da, db, nhits, (R)delta, seed,
+                         b->d_sweep, b->d_counters, b->defect, b->d_cval);
     };
     auto ws_kernel = [&]() { return batch_ws_kernel<R, COMP>(b->ptbc, v); };
     // independent passes two to a launch (bit-identical to one by one, see
@@ -408,7 +479,11 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
                       2LL * b->nblocks <= (long long)ki::cu_count() *
                                               ki::blocks_per_cu(reinterpret_cast<const void *>(ws_kernel()),
                                                                 klft_metro::kWsBlock);
-    if (pair) {
+    if (paired) *paired = pair;
+    if (part > 0) {
+      if (!pair || b->ptbc || !run_parts().ok) return ki::set_error(KLFT_E_STATE, "run parts need paired passes");
+      for (const BLaunch &L : run_parts().part[part - 1]) site_waves(ws_kernel(), L.pa, L.pb, L.da, L.db);
+    } else if (pair) {
       const ki::PassGroups gr = ki::pass_groups(4);
       for (int i = 0; i < gr.n; ++i) site_waves(ws_kernel(), gr.p[i][0], gr.p[i][1]);
     } else {
@@ -425,7 +500,7 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
                              b->defect, b->d_cval);
       }
     }
-    if (b->ptbc) {
+    if (b->ptbc && part == 0) {
       launch_defect_sums<R, COMP>(b->nc, b->d_chains, b->n, b->defect, b->d_A);
       hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, ki::stream(), b->n, b->nc, b->beta[0], b->d_cvals, b->d_A,
                          b->d_conf_of, b->d_cval, b->d_swaps, seed, b->d_sweep);
@@ -440,20 +515,16 @@ int enqueue_step(klft_batch_t b, int nhits, double delta, uint64_t seed) {
     default: rc = passes(ki::tag<float>{}, ki::ic<1>{}); break;
   }
   if (rc) return rc;
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ki::stream(), b->d_sweep);
+  if (part != 1) hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ki::stream(), b->d_sweep);
   KLFT_HIP(hipGetLastError());
   return KLFT_OK;
 }
 
-int ensure_graph(klft_batch_t b, int nhits, double delta, uint64_t seed) {
-  if (b->exec && b->g_nhits == nhits && b->g_delta == delta && b->g_seed == seed) return KLFT_OK;
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
-  prime_occupancy(b);
+// capture part `part` of enqueue_step into a graph and instantiate it
+int capture(klft_batch_t b, int nhits, double delta, uint64_t seed, int part, hipGraph_t *graph,
+            hipGraphExec_t *exec, bool *paired) {
   KLFT_HIP(hipStreamBeginCapture(ki::stream(), hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_step(b, nhits, delta, seed);
+  const int rc = enqueue_step(b, nhits, delta, seed, part, paired);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(ki::stream(), &g);
   if (rc) {
@@ -461,8 +532,29 @@ int ensure_graph(klft_batch_t b, int nhits, double delta, uint64_t seed) {
     return rc;
   }
   if (e != hipSuccess) return ki::hip_fail(e, "graph capture");
-  b->graph = g;
-  KLFT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  *graph = g;
+  KLFT_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return KLFT_OK;
+}
+
+// the graph of one sweep / step and, for chains whose passes pair, the
+// head, window and tail graphs of a run (rebuilt when the parameters change)
+int ensure_graph(klft_batch_t b, int nhits, double delta, uint64_t seed) {
+  if (b->exec && b->g_nhits == nhits && b->g_delta == delta && b->g_seed == seed) return KLFT_OK;
+  drop_graphs(b);
+  b->run_pairs = false;
+  prime_occupancy(b);
+  bool paired = false;
+  int rc = capture(b, nhits, delta, seed, 0, &b->graph, &b->exec, &paired);
+  if (rc) return rc;
+  if (paired && !b->ptbc && run_parts().ok) {
+    for (int i = 0; i < 3 && !rc; ++i) rc = capture(b, nhits, delta, seed, i + 1, &b->rgraph[i], &b->rexec[i], nullptr);
+    if (rc) {
+      drop_graphs(b);
+      return rc;
+    }
+    b->run_pairs = true;
+  }
   b->g_nhits = nhits;
   b->g_delta = delta;
   b->g_seed = seed;
@@ -546,7 +638,14 @@ extern "C" int klft_batch_run(klft_batch_t b, int nhits, double delta, uint64_t 
   if (rc) return rc;
   const uint32_t s = (uint32_t)sweep0;
   KLFT_HIP(hipMemcpyAsync(b->d_sweep, &s, sizeof s, hipMemcpyHostToDevice, ki::stream()));
-  for (int i = 0; i < n; ++i) KLFT_HIP(hipGraphLaunch(b->exec, ki::stream()));
+  if (b->run_pairs && n >= 2) {
+    // passes of consecutive sweeps share launches: head, n - 1 windows, tail
+    KLFT_HIP(hipGraphLaunch(b->rexec[0], ki::stream()));
+    for (int i = 0; i + 1 < n; ++i) KLFT_HIP(hipGraphLaunch(b->rexec[1], ki::stream()));
+    KLFT_HIP(hipGraphLaunch(b->rexec[2], ki::stream()));
+  } else {
+    for (int i = 0; i < n; ++i) KLFT_HIP(hipGraphLaunch(b->exec, ki::stream()));
+  }
   return ki::sync_stream();
 }
 

@@ -225,6 +225,30 @@ def test_ptbc_many_replicas_and_full_length_defect(nrep, nc, dims, L, pos):
     assert list(p.labels()) == conf_of
 
 
+@pytest.mark.parametrize("storage", ["f32", "f32_rows"])
+def test_batch_run_pairs_across_sweeps_at_bench_shape(storage):
+    """16 SU(3) chains of 16^4 (the batched bench leg; passes pair at two
+    blocks per CU): a graph-replayed run of 4 sweeps, whose passes of
+    consecutive sweeps share launches (13|0', 14|1', 15|2'), leaves every
+    chain's links bit for bit as 4 single sweeps do."""
+    chains = [(3, (16, 16, 16, 16), 5.5 + 0.05 * i) for i in range(16)]
+    out = []
+    for mode in ("sweeps", "run"):
+        b = klft.Batch(chains, storage)
+        b.set_identity()
+        if mode == "sweeps":
+            for s in range(4):
+                b.sweep(10, 0.1, 4242, 7 + s)
+        else:
+            b.run(10, 0.1, 4242, 7, 4)
+        out.append(([b.download(i) for i in (0, 5, 15)], b.plaquette()))
+        b.close()
+    for u0, u1 in zip(out[0][0], out[1][0]):
+        assert np.array_equal(u0, u1)
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][1].max() < 1.0
+
+
 @pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("storage", ["f64", "f32", "f32_rows"])
 def test_batch_paired_passes_equal_sequential(monkeypatch, storage, variant):
