@@ -365,9 +365,10 @@ def test_run_pairing_across_sweeps_equals_sequential_sweeps(monkeypatch, storage
         if variant == 15:
             g.set_proposal("symmetric")
         if pairs == "0":
-            rates = [klft.sweep_Metropolis(g, p, s) for s in range(3, 8)]
+            rates = [klft.sweep_Metropolis(g, p, s) for s in range(3, 11)]
         else:
-            rates = list(klft.metropolis_run(g, p, 3, 5))
+            # runs of 1, 2 and 5 sweeps: no window, one, several
+            rates = [r for s0, n in ((3, 1), (4, 2), (6, 5)) for r in klft.metropolis_run(g, p, s0, n)]
         out.append((rates, g.download()))
         g.close()
     assert out[0][0] == out[1][0]
