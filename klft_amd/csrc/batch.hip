@@ -9,7 +9,9 @@
 // whole batch -- counter reset, the 16 passes in 16 or 11 launches and, for
 // replicas, the defect action sums and the swap kernel -- is captured once in
 // a HIP graph and replayed; the sweep / step index lives in device memory and
-// the graph's last node advances it.
+// the graph's last node advances it.  A run of several sweeps of chains that
+// pair their passes replays three more graphs instead (run_parts), so that
+// passes of consecutive sweeps share launches too: 8 per sweep.
 //
 // PTBC (PTBCGaugeField.hpp:30-279): replica r's links on the defect (mu = 0,
 // x0 = 0, x_k in [p_k, p_k + Ld)) are scaled by its coupling c.  Swaps of
