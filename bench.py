@@ -528,9 +528,16 @@ def main():
         tf = traffic / (main_leg["pass_us"] * 1e-6) / 1e12 / (PEAK_HBM_GBS / 1e3)
         roofline["traffic_GBs"] = round(traffic / (main_leg["pass_us"] * 1e-6) / 1e9, 1)
         roofline["traffic_frac"] = round(tf, 4)
+        # the two floors under one pass: its measured bytes at 8 TB/s, and
+        # the time its VALU is busy (VALUBusy x duration, with the VALU
+        # instructions per wave behind it); the larger one is the bound
+        roofline["hbm_traffic_floor_us"] = round(traffic / (PEAK_HBM_GBS * 1e9) * 1e6, 1)
         vb = prof.get("valu_busy_pct")
         if vb is not None:
             roofline["valu_busy_frac"] = round(vb / 100.0, 4)
+            roofline["valu_active_us"] = round(vb / 100.0 * main_leg["pass_us"], 1)
+            if prof.get("valu_insts_per_wave"):
+                roofline["valu_insts_per_wave"] = prof["valu_insts_per_wave"]
             roofline["bound"] = "hbm" if tf >= vb / 100.0 else "valu"
         roofline["profile"] = "profiles/metropolis_pass_su3_32.json"
     cpu = None
