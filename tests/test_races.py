@@ -179,3 +179,21 @@ def test_run_paired_passes_are_race_free(nd, dims):
         merged = visit.copy()
         merged[visit == b] = a
         assert pass_conflicts(nd, dims, merged) == 0, (a, b)
+
+
+def test_run_launches_reports_capacity_and_bad_arguments():
+    """klft_run_launches fills at most `capacity` launches: too small a buffer
+    is an error that still reports the count needed; nd outside 2..4 and a
+    negative sweep count are errors; zero sweeps is zero launches."""
+    import ctypes as C
+    import klft_amd as klft
+    lib = klft.load()
+    n = C.c_int(-1)
+    buf = (C.c_int32 * (4 * 19))()  # capacity counts launches, four ints each
+    assert lib.klft_run_launches(4, 2, buf, 10, C.byref(n)) != 0
+    assert n.value == 19
+    assert lib.klft_run_launches(4, 2, buf, 19, C.byref(n)) == 0 and n.value == 19
+    assert lib.klft_run_launches(5, 2, buf, 10, C.byref(n)) != 0
+    assert lib.klft_run_launches(4, -1, buf, 10, C.byref(n)) != 0
+    assert lib.klft_run_launches(4, 0, None, 0, C.byref(n)) == 0 and n.value == 0
+    assert klft.run_launches(3, 0) == []
