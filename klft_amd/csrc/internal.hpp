@@ -150,8 +150,9 @@ inline PassGroups pass_groups(int nd) {
 // keep their order, and a pass may move ahead of earlier passes it does not
 // touch.  One sweep gives pass_groups' 11 launches in 4D; over many sweeps
 // the first three passes of each sweep pair with the last three of the one
-// before (13|0', 14|1', 15|2', all four parities apart), 8 launches per
-// sweep, every one of them a pair.  sweep[] is relative to the run's first.
+// before (13|0', 14|1', 15|2': three, four and three parities apart), 8
+// launches per sweep, every one of them a pair.  sweep[] is relative to the
+// run's first.
 struct RunLaunch {
   int pass[2];
   int sweep[2];
