@@ -414,7 +414,7 @@ struct BLaunch {
 // relative to the device sweep index, which the window then advances by
 // one; tail = the launches after the last window ({13} {14} {15}), then an
 // advance.  The run is head, n - 1 windows, tail.  ok is false if the
-// schedule does not repeat with period one sweep (it does for every nd).
+// schedule did not repeat with period one sweep (in 4D it does).
 struct RunParts {
   std::vector<BLaunch> part[3];
   bool ok = false;
@@ -429,10 +429,10 @@ const RunParts &run_parts() {
     auto conv = [](const ki::RunLaunch &L, int base) {
       return BLaunch{L.pass[0], L.sweep[0] - base, L.pass[1], L.pass[1] >= 0 ? L.sweep[1] - base : 0};
     };
-    size_t k1 = 0, k2;
+    size_t k1 = 0;
     while (k1 < S.size() && top(S[k1]) < 1) ++k1;
-    for (k2 = k1; k2 < S.size() && top(S[k2]) < 2; ++k2) {
-    }
+    size_t k2 = k1;
+    while (k2 < S.size() && top(S[k2]) < 2) ++k2;
     const size_t w = k2 - k1;
     if (w == 0 || k2 + w > S.size()) return r;
     for (size_t i = 0; i < k1; ++i) r.part[0].push_back(conv(S[i], 0));
