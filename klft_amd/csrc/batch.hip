@@ -407,14 +407,11 @@ struct BLaunch {
   int pa, da, pb, db;
 };
 // A run of n >= 2 sweeps of chains, with passes of consecutive sweeps
-// sharing launches (run_schedule): head = sweep 0's launches before its
-// first pair with sweep 1 (4D: {0} {1} {2} {3,4} {5,8} {6,9} {7,10}
-// {11,12}); window = the launches from there up to the first that holds a
-// pass of sweep 2 ({13,0'} {14,1'} {15,2'} {3',4'} ... {11',12'}), offsets
-// relative to the device sweep index, which the window then advances by
-// one; tail = the launches after the last window ({13} {14} {15}), then an
-// advance.  The run is head, n - 1 windows, tail.  ok is false if the
-// schedule did not repeat with period one sweep (in 4D it does).
+// sharing launches: the head, window and tail of run_period(4) as three
+// graphs.  The head's sweep offsets are relative to the device sweep index;
+// the window's too, and the window then advances that index by one; the
+// tail's are relative to the last sweep, then an advance.  The run is head,
+// n - 1 windows, tail.
 struct RunParts {
   std::vector<BLaunch> part[3];
   bool ok = false;
@@ -422,26 +419,14 @@ struct RunParts {
 const RunParts &run_parts() {
   static const RunParts rp = [] {
     RunParts r;
-    const std::vector<ki::RunLaunch> S = ki::run_schedule(4, 3);
-    auto top = [](const ki::RunLaunch &L) {
-      return L.pass[1] >= 0 && L.sweep[1] > L.sweep[0] ? L.sweep[1] : L.sweep[0];
+    const ki::RunPeriod per = ki::run_period(4);
+    if (!per.ok) return r;
+    auto conv = [](const ki::RunLaunch &L) {
+      return BLaunch{L.pass[0], L.sweep[0], L.pass[1], L.pass[1] >= 0 ? L.sweep[1] : 0};
     };
-    auto conv = [](const ki::RunLaunch &L, int base) {
-      return BLaunch{L.pass[0], L.sweep[0] - base, L.pass[1], L.pass[1] >= 0 ? L.sweep[1] - base : 0};
-    };
-    size_t k1 = 0;
-    while (k1 < S.size() && top(S[k1]) < 1) ++k1;
-    size_t k2 = k1;
-    while (k2 < S.size() && top(S[k2]) < 2) ++k2;
-    const size_t w = k2 - k1;
-    if (w == 0 || k2 + w > S.size()) return r;
-    for (size_t i = 0; i < k1; ++i) r.part[0].push_back(conv(S[i], 0));
-    for (size_t i = k1; i < k2; ++i) r.part[1].push_back(conv(S[i], 0));
-    for (size_t i = 0; i < w; ++i) {
-      const BLaunch a = conv(S[k2 + i], 1), c = r.part[1][i];
-      if (a.pa != c.pa || a.da != c.da || a.pb != c.pb || a.db != c.db) return r;
-    }
-    for (size_t i = k2 + w; i < S.size(); ++i) r.part[2].push_back(conv(S[i], 2));
+    for (const ki::RunLaunch &L : per.head) r.part[0].push_back(conv(L));
+    for (const ki::RunLaunch &L : per.window) r.part[1].push_back(conv(L));
+    for (const ki::RunLaunch &L : per.tail) r.part[2].push_back(conv(L));
     r.ok = true;
     return r;
   }();

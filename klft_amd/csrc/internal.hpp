@@ -184,6 +184,74 @@ inline std::vector<RunLaunch> run_schedule(int nd, int nsweeps) {
   return out;
 }
 
+// run_schedule is periodic with a period of one sweep: for n >= 2 sweeps it
+// is a head (the first sweep's launches before its first pair with the
+// second), n - 1 windows (window i: the launches holding the last passes of
+// sweep i and the first of sweep i + 1, then the rest of sweep i + 1 up to
+// its pairs with sweep i + 2) and a tail (the last sweep's remaining passes).
+// run_period derives the three from run_schedule(nd, 3) and checks that the
+// second window is the first one sweep later; sweeps in `window` are
+// relative to the window's index, in `tail` to the last sweep.  4D: head
+// {0} {1} {2} {3,4} {5,8} {6,9} {7,10} {11,12}, window {13,0'} {14,1'}
+// {15,2'} {3',4'} {5',8'} {6',9'} {7',10'} {11',12'}, tail {13} {14} {15}.
+struct RunPeriod {
+  std::vector<RunLaunch> head, window, tail;
+  bool ok = false;
+};
+inline RunPeriod run_period(int nd) {
+  RunPeriod r;
+  const std::vector<RunLaunch> S = run_schedule(nd, 3);
+  auto top = [](const RunLaunch &L) { return L.pass[1] >= 0 && L.sweep[1] > L.sweep[0] ? L.sweep[1] : L.sweep[0]; };
+  auto shifted = [](RunLaunch L, int by) {
+    L.sweep[0] -= by;
+    if (L.pass[1] >= 0) L.sweep[1] -= by;
+    return L;
+  };
+  size_t k1 = 0;
+  while (k1 < S.size() && top(S[k1]) < 1) ++k1;
+  size_t k2 = k1;
+  while (k2 < S.size() && top(S[k2]) < 2) ++k2;
+  const size_t w = k2 - k1;
+  if (w == 0 || k2 + w > S.size()) return r;
+  r.head.assign(S.begin(), S.begin() + (long)k1);
+  r.window.assign(S.begin() + (long)k1, S.begin() + (long)k2);
+  for (size_t i = 0; i < w; ++i) {
+    const RunLaunch a = shifted(S[k2 + i], 1), &c = r.window[i];
+    if (a.pass[0] != c.pass[0] || a.sweep[0] != c.sweep[0] || a.pass[1] != c.pass[1] ||
+        (a.pass[1] >= 0 && a.sweep[1] != c.sweep[1]))
+      return r;
+  }
+  for (size_t i = k2 + w; i < S.size(); ++i) r.tail.push_back(shifted(S[i], 2));
+  r.ok = true;
+  return r;
+}
+// the launches of a run of n sweeps, in order, without building the list:
+// fn(launch) with sweeps relative to the run's first (the period above,
+// cached per nd; one sweep: run_schedule(nd, 1))
+template <typename Fn>
+inline void for_each_run_launch(int nd, int nsweeps, Fn &&fn) {
+  if (nsweeps <= 0) return;
+  if (nsweeps == 1) {
+    for (const RunLaunch &L : run_schedule(nd, 1)) fn(L);
+    return;
+  }
+  static const RunPeriod periods[3] = {run_period(2), run_period(3), run_period(4)};
+  const RunPeriod &rp = periods[nd - 2];
+  if (!rp.ok) {
+    for (const RunLaunch &L : run_schedule(nd, nsweeps)) fn(L);
+    return;
+  }
+  auto at = [](RunLaunch L, int base) {
+    L.sweep[0] += base;
+    if (L.pass[1] >= 0) L.sweep[1] += base;
+    return L;
+  };
+  for (const RunLaunch &L : rp.head) fn(L);
+  for (int i = 0; i + 1 < nsweeps; ++i)
+    for (const RunLaunch &L : rp.window) fn(at(L, i));
+  for (const RunLaunch &L : rp.tail) fn(at(L, nsweeps - 1));
+}
+
 // compute units of the current device (256 on an MI355X), queried once
 int cu_count();
 // workgroups of `block` threads of kernel `kern` resident per CU at once

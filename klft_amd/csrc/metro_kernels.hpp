@@ -498,8 +498,9 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
       pair = 2LL * nws <= (long long)cu_count() * blocks_per_cu(kern, kWsBlock);
     }
     if (pair) {
-      const std::vector<RunLaunch> sched = run_schedule(ND, nsweeps);
-      for (const RunLaunch &L : sched) {
+      int rc = KLFT_OK;
+      for_each_run_launch(ND, nsweeps, [&](const RunLaunch &L) {
+        if (rc) return;
         const int pa = L.pass[0], pb = L.pass[1];
         char label[56];
         if (pb >= 0)
@@ -507,9 +508,9 @@ int launch_passes_typed(klft_field_t f, void *links, int variant, int p0, int p1
         else
           snprintf(label, sizeof label, "sweep_Metropolis_GaugeField_sublat_%d", pa);  // Metropolis.hpp:161
         ProfScope prof(pb >= 0 ? "metropolis_pass_pair" : "metropolis_pass", label);
-        const int rc = launch_ws(pa, L.sweep[0], pb, pb >= 0 ? L.sweep[1] : L.sweep[0]);
-        if (rc) return rc;
-      }
+        rc = launch_ws(pa, L.sweep[0], pb, pb >= 0 ? L.sweep[1] : L.sweep[0]);
+      });
+      if (rc) return rc;
     } else {
       for (int i = 0; i < nsweeps; ++i) {
         for (int pass = p0; pass < p1; ++pass) {

@@ -219,16 +219,19 @@ extern "C" int klft_run_launches(int nd, int nsweeps, int32_t *launches, int cap
   if (nd < 2 || nd > 4) return ki::set_error(KLFT_E_ARG, "nd=%d unsupported (2..4)", nd);
   if (nsweeps < 0) return ki::set_error(KLFT_E_ARG, "negative sweep count");
   if (!nlaunches || (capacity > 0 && !launches)) return ki::set_error(KLFT_E_ARG, "null argument");
-  const std::vector<ki::RunLaunch> sched = ki::run_schedule(nd, nsweeps);
-  *nlaunches = (int)sched.size();
-  if ((int)sched.size() > capacity)
-    return ki::set_error(KLFT_E_ARG, "%d launches do not fit a capacity of %d", (int)sched.size(), capacity);
-  for (size_t i = 0; i < sched.size(); ++i) {
-    launches[4 * i] = sched[i].pass[0];
-    launches[4 * i + 1] = sched[i].sweep[0];
-    launches[4 * i + 2] = sched[i].pass[1];
-    launches[4 * i + 3] = sched[i].pass[1] >= 0 ? sched[i].sweep[1] : -1;
-  }
+  // the same walk over the periodic schedule that the sweep launches follow
+  long long n = 0;
+  ki::for_each_run_launch(nd, nsweeps, [&](const ki::RunLaunch &L) {
+    if (n < capacity) {
+      launches[4 * n] = L.pass[0];
+      launches[4 * n + 1] = L.sweep[0];
+      launches[4 * n + 2] = L.pass[1];
+      launches[4 * n + 3] = L.pass[1] >= 0 ? L.sweep[1] : -1;
+    }
+    ++n;
+  });
+  *nlaunches = (int)n;
+  if (n > capacity) return ki::set_error(KLFT_E_ARG, "%lld launches do not fit a capacity of %d", n, capacity);
   return KLFT_OK;
 }
 

@@ -197,3 +197,31 @@ def test_run_launches_reports_capacity_and_bad_arguments():
     assert lib.klft_run_launches(4, -1, buf, 10, C.byref(n)) != 0
     assert lib.klft_run_launches(4, 0, None, 0, C.byref(n)) == 0 and n.value == 0
     assert klft.run_launches(3, 0) == []
+
+
+def greedy_run_schedule(nd, nsweeps):
+    """The run schedule by its definition: passes in program order, each into
+    the first launch after the last pass it conflicts with (parities at most
+    two apart) that holds fewer than two passes."""
+    launches, at = [], {}
+    order = [(p, s) for s in range(nsweeps) for p in range(1 << nd)]
+    for i, (p, s) in enumerate(order):
+        lo = max([at[q] + 1 for q in order[:i] if bin(p ^ q[0]).count("1") <= 2], default=0)
+        k = lo
+        while k < len(launches) and len(launches[k]) >= 2:
+            k += 1
+        if k == len(launches):
+            launches.append([])
+        launches[k].append((p, s))
+        at[(p, s)] = k
+    return [tuple(launch) for launch in launches]
+
+
+@pytest.mark.parametrize("nd", [2, 3, 4])
+@pytest.mark.parametrize("nsweeps", [1, 2, 3, 7])
+def test_run_launches_equal_the_greedy_schedule(nd, nsweeps):
+    """The library walks the run schedule as a head, one window per further
+    sweep and a tail (its period of one sweep); the walk equals the greedy
+    schedule built pass by pass from its definition."""
+    import klft_amd as klft
+    assert klft.run_launches(nd, nsweeps) == greedy_run_schedule(nd, nsweeps)
