@@ -375,6 +375,27 @@ def test_run_pairing_across_sweeps_equals_sequential_sweeps(monkeypatch, storage
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("storage,nc", [("f32", 2), ("f64", 3)])
+def test_long_run_equals_sequential_sweeps(monkeypatch, storage, nc):
+    """A run of 300 sweeps (head, 299 windows of the one-sweep period, tail)
+    leaves the links and all 300 acceptance rates bit for bit as 300 single
+    sweeps with unpaired passes do."""
+    dims = (4, 4, 4, 4)
+    p = klft.MetropolisParams(nHits=4, beta={2: 2.3, 3: 5.5}[nc], delta=0.2, seed=77)
+    out = []
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("KLFT_VARIANT", f"metropolis_pairs={pairs}")
+        g = klft.GaugeField(4, nc, dims, storage).set_random_algebra(0.5, 9)
+        if pairs == "0":
+            rates = [klft.sweep_Metropolis(g, p, s) for s in range(300)]
+        else:
+            rates = list(klft.metropolis_run(g, p, 0, 300))
+        out.append((rates, g.download()))
+        g.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32", "f32_rows"])
 def test_new_field_links_start_zeroed(storage):
     """A field's links read back as zeros before any initialiser runs."""
